@@ -1,0 +1,226 @@
+/*
+ * smip_mi355x.h -- C ABI of the MI355X speckle-masking engine (libsmip_mi355x.so).
+ *
+ * This is the drop-in boundary for the hot path of hangeza/speckle-stacker 0.3.2
+ * (SMIP).  The reference exposes that path as header-only C++ templates; each
+ * entry point below replaces one of its members (file:line cited per
+ * declaration).  The C++ mirror of the reference surface (namespace smip,
+ * speckle-stacker_amd/smip/ headers) is written on top of these calls.
+ *
+ * Conventions
+ *  - every function returns a status code (SMIP_OK = 0); the message of the last
+ *    failure on the calling thread is returned by smip_last_error();
+ *  - host arrays are plain pointers: complex values are interleaved (re, im),
+ *    2-D arrays are row-major with x (column) fastest, the layout of
+ *    smip::Array2 (include/array2.h:290-308);
+ *  - the bispectrum is stored on the device in the reference layout
+ *    [-ux][uy wrapped][-vx][vy wrapped] (include/bispectrum.h:324-349) so that
+ *    exported buffers and .dat files are interchangeable with the reference's;
+ *  - one HIP stream per job; calls on one object are not thread-safe, distinct
+ *    objects may be used from distinct threads.
+ */
+#ifndef SMIP_MI355X_H
+#define SMIP_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define SMIP_API __attribute__((visibility("default")))
+#else
+#define SMIP_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: one per exception class thrown by the reference path. */
+#define SMIP_OK 0
+#define SMIP_ERR_INVALID_ARGUMENT 1  /* std::invalid_argument */
+#define SMIP_ERR_OUT_OF_RANGE 2      /* std::out_of_range */
+#define SMIP_ERR_RUNTIME 3           /* std::runtime_error */
+#define SMIP_ERR_BAD_FUNCTION_CALL 4 /* std::bad_function_call */
+#define SMIP_ERR_LOGIC 5             /* std::logic_error */
+#define SMIP_ERR_DOMAIN 6            /* std::domain_error */
+#define SMIP_ERR_HIP 7               /* HIP runtime failure */
+#define SMIP_ERR_NO_MEMORY 8         /* device memory exhausted or admission refused */
+
+/* Sample and element types. */
+#define SMIP_DTYPE_U8 1
+#define SMIP_DTYPE_U16 2
+#define SMIP_DTYPE_F32 3
+#define SMIP_DTYPE_F64 4
+#define SMIP_DTYPE_C64 5  /* std::complex<float>, bispec_complex_t (include/types.h:33) */
+#define SMIP_DTYPE_C128 6 /* std::complex<double>, complex_t (include/types.h:32) */
+
+/* Arithmetic operators of Array_base / Bispectrum. */
+#define SMIP_OP_ADD 0
+#define SMIP_OP_SUB 1
+#define SMIP_OP_MUL 2
+#define SMIP_OP_DIV 3
+
+/* Accumulation precision of an engine job. */
+#define SMIP_PRECISION_FP64 0  /* plain drop-in: FP64 FFT, per-frame FP64 triple products */
+#define SMIP_PRECISION_MIXED 1 /* batched: FP32 FFT + triple products per batch, FP64 fold */
+
+/* Thread-local message of the last failing call (empty string if none). */
+SMIP_API const char* smip_last_error(void);
+/* Library version; the reference's is config/config.h:38-50 (0.3.2). */
+SMIP_API int smip_version(int32_t* major, int32_t* minor, int32_t* patch);
+
+/* ------------------------------------------------------------------------ */
+/* Context: one GPU                                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_ctx smip_ctx;
+SMIP_API int smip_ctx_create(int32_t device, smip_ctx** out);
+SMIP_API int smip_ctx_destroy(smip_ctx* ctx);
+SMIP_API int smip_ctx_synchronize(smip_ctx* ctx);
+SMIP_API int smip_ctx_mem_info(smip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ------------------------------------------------------------------------ */
+/* smip::Bispectrum<T> (include/bispectrum.h:28-125)                        */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_bispec smip_bispec;
+
+typedef struct smip_bispec_info {
+    uint64_t dims[4];       /* constructor extents {ncols, nrows, b0, b1} */
+    uint64_t sizes[4];      /* sizes(), :85-86,203-211 */
+    uint64_t base_sizes[4]; /* base_sizes(), :87,213-220 */
+    uint64_t base_size;     /* base_size() == size(), :85,88 */
+    uint64_t totalsize;     /* totalsize(), :89 */
+    int32_t min_indices[4]; /* min_indices(), :90,151-153 */
+    int32_t max_indices[4]; /* max_indices(), :91,154-158 */
+    int32_t dtype;          /* SMIP_DTYPE_C64 or SMIP_DTYPE_C128 */
+} smip_bispec_info;
+
+/* Bispectrum(const extents&) zero-filled, include/bispectrum.h:182-192; all-zero
+ * dims give the default-constructed empty bispectrum (:176-180). */
+SMIP_API int smip_bispec_create(smip_ctx* ctx, const uint64_t dims[4], int32_t dtype, smip_bispec** out);
+SMIP_API int smip_bispec_destroy(smip_bispec* b);
+/* sizes(), base_sizes(), min_indices() ..., include/bispectrum.h:85-91 */
+SMIP_API int smip_bispec_info_get(const smip_bispec* b, smip_bispec_info* out);
+/* accumulate_from_fft(const Array2<U>&), include/bispectrum.h:356-409.  fft is the
+ * nrows x ncols complex<double> Array2 in FFTW output order. */
+SMIP_API int smip_bispec_accumulate_from_fft(smip_bispec* b, const double* fft, uint64_t ncols, uint64_t nrows);
+/* get_elements(idx_list, results), include/bispectrum.h:514-523 (get_element
+ * :482-512 for n == 1): indices are n x 4 ints, values n complex doubles. */
+SMIP_API int smip_bispec_get_elements(const smip_bispec* b, const int32_t* indices, uint64_t n, double* values);
+/* put_element, include/bispectrum.h:525-535 */
+SMIP_API int smip_bispec_put_element(smip_bispec* b, const int32_t idx[4], const double value[2]);
+/* calc_offset (no range check, modulo 2^64), include/bispectrum.h:318-349 */
+SMIP_API uint64_t smip_bispec_calc_offset(const smip_bispec* b, const int32_t idx[4]);
+/* calc_indices, include/bispectrum.h:280-316 */
+SMIP_API int smip_bispec_calc_indices(const smip_bispec* b, uint64_t addr, int32_t idx[4]);
+/* Array_base<T>::data() contents (base_size elements of the bispectrum's dtype). */
+SMIP_API int smip_bispec_download(const smip_bispec* b, void* host);
+SMIP_API int smip_bispec_upload(smip_bispec* b, const void* host);
+/* write_to_file / read_from_file, include/bispectrum.h:411-462 (same byte format). */
+SMIP_API int smip_bispec_write_file(const smip_bispec* b, const char* path);
+SMIP_API int smip_bispec_read_file(smip_bispec* b, const char* path);
+/* Array_base operator op=(value), include/array_base.h:300-349 (/= 0 throws). */
+SMIP_API int smip_bispec_scalar_op(smip_bispec* b, int32_t op, double re, double im);
+/* Bispectrum operator op=(const Bispectrum&), include/bispectrum.h:232-278. */
+SMIP_API int smip_bispec_array_op(smip_bispec* b, const smip_bispec* other, int32_t op);
+/* operator=(const Bispectrum&), include/bispectrum.h:222-230 (dst keeps its dtype). */
+SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* src);
+
+/* ------------------------------------------------------------------------ */
+/* Phase reconstruction (include/phasereco.h:15-27, src/phasereco.cpp)      */
+/* ------------------------------------------------------------------------ */
+/* reconstruct_phases<complex<double>, U>(bispec, xsize, ysize, reco_radius,
+ * &phasemap), include/phasereco.h:33-73.  phases: xsize*ysize complex doubles;
+ * flags / consistency: the PhaseMapElement fields (include/phasemap.h:10-20).
+ * Any output pointer may be NULL. */
+SMIP_API int smip_reconstruct_phases(const smip_bispec* b, uint64_t xsize, uint64_t ysize, double reco_radius,
+    double* phases, uint8_t* flags, double* consistency);
+/* NextRecoIndex, src/phasereco.cpp:17-36 (host function). */
+SMIP_API void smip_next_reco_index(double* r, double* phi, int32_t* i, int32_t* j);
+
+/* ------------------------------------------------------------------------ */
+/* smip::CrossCorrelation<double> (include/crosscorrel.h:43-189)            */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_xcorr smip_xcorr;
+/* CrossCorrelation(const Array2<T>& ref), :89-94 */
+SMIP_API int smip_xcorr_create(smip_ctx* ctx, const double* ref, uint64_t ncols, uint64_t nrows, smip_xcorr** out);
+SMIP_API int smip_xcorr_destroy(smip_xcorr* x);
+/* correlate(frame), :96-143 (size mismatch -> SMIP_ERR_INVALID_ARGUMENT) */
+SMIP_API int smip_xcorr_correlate(smip_xcorr* x, const double* frame, uint64_t ncols, uint64_t nrows);
+/* get_correlation_array(), :145-152 (before correlate -> SMIP_ERR_BAD_FUNCTION_CALL) */
+SMIP_API int smip_xcorr_get_correlation(smip_xcorr* x, double* out);
+/* get_displacement(), :154-164 */
+SMIP_API int smip_xcorr_get_displacement(smip_xcorr* x, int32_t shift[2]);
+/* operator()(frame) = correlate + displacement, :181-189 */
+SMIP_API int smip_xcorr_displacement(smip_xcorr* x, const double* frame, uint64_t ncols, uint64_t nrows,
+    int32_t shift[2]);
+
+/* ------------------------------------------------------------------------ */
+/* Window functions (include/window_function.h:11-84), host                 */
+/* ------------------------------------------------------------------------ */
+/* GeneralHamming(xsize, ysize, aperture, alpha); Hann is alpha = 0.5,
+ * Hamming alpha = 0.54.  aperture <= 0 -> SMIP_ERR_DOMAIN.  out: real values. */
+SMIP_API int smip_general_hamming(uint64_t xsize, uint64_t ysize, double aperture, double alpha, double* out);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident frame stacks                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_frames smip_frames;
+SMIP_API int smip_frames_create(smip_ctx* ctx, int32_t dtype, uint64_t ncols, uint64_t nrows, uint64_t count,
+    smip_frames** out);
+SMIP_API int smip_frames_destroy(smip_frames* f);
+SMIP_API int smip_frames_upload(smip_frames* f, uint64_t first, uint64_t count, const void* host);
+
+/* ------------------------------------------------------------------------ */
+/* Engine jobs: the smip-cli per-frame loop and post-processing            */
+/* (smip-cli/main.cpp:242-376) on device-resident state                    */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_job smip_job;
+
+typedef struct smip_job_params {
+    uint64_t ncols, nrows;     /* frame extents (powers of two, 8..4096) */
+    uint64_t bdepth;           /* -b, bispectrum extent of dims 3 and 4 */
+    double reco_radius;        /* -p */
+    int32_t frame_dtype;       /* SMIP_DTYPE_U8 / U16 / F32 */
+    int32_t precision;         /* SMIP_PRECISION_FP64 or SMIP_PRECISION_MIXED */
+    int32_t saa;               /* 1: cross-correlation shift-and-add against frame 0 */
+    int32_t frames_per_batch;  /* FP32 frames per FP64 fold (MIXED); 0 = default */
+    int32_t vy_tile;           /* vy values per register tile (MIXED); 0 = default */
+} smip_job_params;
+
+typedef struct smip_job_stats {
+    uint64_t frames;        /* frames accumulated */
+    double accumulate_ms;   /* device time of FFT + |F|^2 + triple products */
+    double saa_ms;          /* device time of the shift-and-add stage */
+    double reco_ms;         /* device time of phase recovery */
+    double synth_ms;        /* device time of image synthesis */
+    uint64_t footprint;     /* device bytes held by the job */
+    uint64_t reco_calls;    /* calc_phase invocations */
+    uint64_t cells_per_frame;
+} smip_job_stats;
+
+/* Device bytes a job of these parameters holds (admission control). */
+SMIP_API int smip_job_footprint(const smip_job_params* p, uint64_t* bytes);
+SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p, smip_job** out);
+SMIP_API int smip_job_destroy(smip_job* j);
+/* Enqueue frames [first, first+count) of a device stack (asynchronous). */
+SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, uint64_t first, uint64_t count);
+/* Upload host frames and enqueue them. */
+SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t count);
+/* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376). */
+SMIP_API int smip_job_finalize(smip_job* j);
+SMIP_API int smip_job_synchronize(smip_job* j);
+SMIP_API int smip_job_get_stats(smip_job* j, smip_job_stats* out);
+/* The job's bispectrum as a Bispectrum object (borrowed; valid until the job is
+ * destroyed).  After finalize it holds the normalised values. */
+SMIP_API int smip_job_bispectrum(smip_job* j, smip_bispec** out);
+/* Outputs after finalize, each nrows x ncols: */
+SMIP_API int smip_job_get_powerspec(smip_job* j, double* out);        /* Re of powerspec / (n N^2) */
+SMIP_API int smip_job_get_sum_image(smip_job* j, double* out);        /* SAA sum / n */
+SMIP_API int smip_job_get_shifts(smip_job* j, int32_t* out, uint64_t count); /* count x 2 */
+SMIP_API int smip_job_get_phases(smip_job* j, double* phases, uint8_t* flags, double* consistency);
+SMIP_API int smip_job_get_image(smip_job* j, double* out);            /* complex, /max|.| */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

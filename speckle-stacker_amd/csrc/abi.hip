@@ -1,0 +1,1309 @@
+// abi.cpp -- C ABI of libsmip_mi355x.so (declared in include/smip_mi355x.h).
+//
+// Host side of the engine: object lifetimes, the job pipeline (the smip-cli
+// per-frame loop and post-processing of smip-cli/main.cpp:242-376 on
+// device-resident state), the spiral visit list (src/phasereco.cpp:17-36), the
+// window function (include/window_function.h:69-84) and the mapping of C++
+// exceptions to status codes.  All arithmetic on frames, spectra and the
+// bispectrum runs in the HIP kernels of fft.hip, bispec.hip, reco.hip, post.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "smip_internal.h"
+
+using namespace smip_amd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& fn)
+{
+    try {
+        fn();
+        g_last_error.clear();
+        return SMIP_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return SMIP_ERR_NO_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SMIP_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return SMIP_ERR_RUNTIME;
+    }
+}
+
+void require(bool cond, int code, const char* msg)
+{
+    if (!cond)
+        fail(code, msg);
+}
+
+template <typename T>
+T* dev_alloc(size_t n)
+{
+    if (n == 0)
+        return nullptr;
+    T* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(SMIP_ERR_NO_MEMORY, "device allocation of " + std::to_string(sizeof(T) * n) + " bytes failed");
+    }
+    return p;
+}
+
+struct DevFree {
+    void operator()(void* p) const
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+};
+template <typename T>
+using dptr = std::unique_ptr<T, DevFree>;
+
+template <typename T>
+dptr<T> dmake(size_t n) { return dptr<T>(dev_alloc<T>(n)); }
+
+constexpr double k2pi = 3.14159265358979323846 * 2.0; // constants::c_2pi<double>
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+struct smip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct smip_bispec {
+    smip_ctx* ctx = nullptr;
+    Desc d {};
+    int dtype = SMIP_DTYPE_C128;
+    void* data = nullptr; // device, base_size elements
+    bool borrowed = false; // reference-backed (a job's accumulator)
+    size_t esz() const { return dtype == SMIP_DTYPE_C64 ? sizeof(float2) : sizeof(double2); }
+    ~smip_bispec()
+    {
+        if (!borrowed && data)
+            (void)hipFree(data);
+    }
+};
+
+struct smip_xcorr {
+    smip_ctx* ctx = nullptr;
+    int X = 0, Y = 0;
+    FftPlan* plan = nullptr;
+    dptr<double> ref, frame, corr;
+    dptr<double2> ref_spec, spec, work;
+    dptr<int32_t> shift_dev;
+    int readiness = 0; // 0 none, 1 correl, 2 shift
+    int32_t shift[2] = { 0, 0 };
+    ~smip_xcorr() { fft_plan_destroy(plan); }
+};
+
+struct smip_frames {
+    smip_ctx* ctx = nullptr;
+    int dtype = 0;
+    uint64_t X = 0, Y = 0, count = 0;
+    void* data = nullptr;
+    ~smip_frames()
+    {
+        if (data)
+            (void)hipFree(data);
+    }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0; // 0 accumulate, 1 saa, 2 reco, 3 synth
+};
+
+struct smip_job {
+    smip_ctx* ctx = nullptr;
+    smip_job_params p {};
+    hipStream_t stream = nullptr;
+    Desc d {};
+    LoopBounds L {};
+    HalfPlane hp {};
+    FftPlan* plan = nullptr;
+    TilePlan* tile = nullptr;
+    int batch = 0;
+    dptr<double2> B64;
+    dptr<double> P_half;
+    dptr<float2> H32, work32;
+    dptr<double2> H64, work64;
+    // shift-and-add
+    dptr<double2> ref_spec, sspec, swork;
+    dptr<double> corr, sum;
+    dptr<int32_t> shifts;
+    uint64_t shifts_cap = 0;
+    // host upload staging
+    dptr<uint8_t> upload;
+    // outputs
+    dptr<double2> phases, image;
+    dptr<uint8_t> flags;
+    dptr<double> cons, scratch;
+    uint64_t nframes = 0;
+    bool finalized = false;
+    uint64_t reco_calls = 0;
+    std::unique_ptr<smip_bispec> view;
+    std::vector<EventPair> events;
+    double ms[4] = { 0, 0, 0, 0 };
+    uint64_t footprint = 0;
+    ~smip_job()
+    {
+        for (auto& e : events) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        tile_plan_destroy(tile);
+        fft_plan_destroy(plan);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+namespace smip_amd {
+size_t dtype_size(int dtype)
+{
+    switch (dtype) {
+    case SMIP_DTYPE_U8:
+        return 1;
+    case SMIP_DTYPE_U16:
+        return 2;
+    case SMIP_DTYPE_F32:
+        return 4;
+    case SMIP_DTYPE_F64:
+        return 8;
+    case SMIP_DTYPE_C64:
+        return 8;
+    case SMIP_DTYPE_C128:
+        return 16;
+    default:
+        fail(SMIP_ERR_INVALID_ARGUMENT, "unknown dtype");
+    }
+}
+} // namespace smip_amd
+
+namespace {
+
+// NextRecoIndex, src/phasereco.cpp:17-36
+void next_reco_index(double& r, double& phi, int32_t& i, int32_t& j)
+{
+    constexpr double dr = 1.0;
+    const double dphi = 1.0 / (k2pi * r);
+    int32_t ii = i, jj = j;
+    while (i == ii && j == jj) {
+        if ((phi += dphi) > k2pi) {
+            phi = 0;
+            r += dr;
+        }
+        ii = static_cast<int32_t>(r * std::cos(phi));
+        jj = static_cast<int32_t>(r * std::sin(phi));
+    }
+    i = ii;
+    j = jj;
+}
+
+// Visits of reconstruct_phases' loop (include/phasereco.h:56-68) that fall in
+// the PhaseMap range; the others are no-ops.
+std::vector<int32_t> visit_list(double reco_radius, int64_t X, int64_t Y)
+{
+    const int32_t mn0 = -(int32_t)(X >> 1), mn1 = -(int32_t)(Y >> 1);
+    const int32_t mx0 = -(int32_t)X / 2 + (int32_t)X - 1, mx1 = -(int32_t)Y / 2 + (int32_t)Y - 1;
+    std::vector<int32_t> v;
+    double r = 0.0, phi = 0.0;
+    int32_t i = 0, j = 0;
+    while (r <= reco_radius) {
+        next_reco_index(r, phi, i, j);
+        if (i >= mn0 && i <= mx0 && j >= mn1 && j <= mx1) {
+            v.push_back(i);
+            v.push_back(j);
+        }
+    }
+    return v;
+}
+
+void check_reco_geometry(const Desc& d, uint64_t xsize, uint64_t ysize)
+{
+    require(xsize > 0 && ysize > 0, SMIP_ERR_INVALID_ARGUMENT, "phase map size must be positive");
+    require(xsize * ysize <= (1u << 22), SMIP_ERR_INVALID_ARGUMENT, "phase map too large");
+    // pm.at(v) must stay inside the PhaseMap for every v of the stored range
+    require((uint64_t)std::max(d.max_idx[2], 0) < xsize && (uint64_t)std::max(d.max_idx[3], 0) < ysize,
+        SMIP_ERR_INVALID_ARGUMENT, "bispectrum v range exceeds the phase map");
+}
+
+__global__ void k_scatter_half(const double2* __restrict__ full, int32_t X, int32_t Y, HalfPlane hp,
+    double2* __restrict__ H)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)hp.hx * Y)
+        return;
+    const int32_t c = (int32_t)(i / Y), r = (int32_t)(i - (int64_t)c * Y);
+    const int32_t ux = c - (X >> 1), uy = r - (Y >> 1);
+    const int32_t col = ux < 0 ? ux + X : ux, row = uy < 0 ? uy + Y : uy;
+    H[(int64_t)c * hp.rs + hp.pad + r] = full[(int64_t)row * X + col];
+}
+
+template <typename V>
+__global__ void k_array_op(V* __restrict__ a, const V* __restrict__ b, uint64_t n, int op)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    V x = a[i];
+    const V y = b[i];
+    switch (op) {
+    case SMIP_OP_ADD:
+        x.x += y.x;
+        x.y += y.y;
+        break;
+    case SMIP_OP_SUB:
+        x.x -= y.x;
+        x.y -= y.y;
+        break;
+    case SMIP_OP_MUL: {
+        const auto re = x.x * y.x - x.y * y.y, im = x.x * y.y + x.y * y.x;
+        x.x = re;
+        x.y = im;
+        break;
+    }
+    default: {
+        const auto den = y.x * y.x + y.y * y.y;
+        const auto re = (x.x * y.x + x.y * y.y) / den, im = (x.y * y.x - x.x * y.y) / den;
+        x.x = re;
+        x.y = im;
+    }
+    }
+    a[i] = x;
+}
+
+__global__ void k_frame_to_f64(const void* __restrict__ in, int dtype, int64_t n, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double v;
+    switch (dtype) {
+    case SMIP_DTYPE_U8:
+        v = (double)static_cast<const uint8_t*>(in)[i];
+        break;
+    case SMIP_DTYPE_U16:
+        v = (double)static_cast<const uint16_t*>(in)[i];
+        break;
+    case SMIP_DTYPE_F32:
+        v = (double)static_cast<const float*>(in)[i];
+        break;
+    default:
+        v = static_cast<const double*>(in)[i];
+    }
+    out[i] = v;
+}
+
+void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// misc
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API const char* smip_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" SMIP_API int smip_version(int32_t* major, int32_t* minor, int32_t* patch)
+{
+    return guarded([&] {
+        require(major && minor && patch, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        *major = 0;
+        *minor = 3;
+        *patch = 2;
+    });
+}
+
+extern "C" SMIP_API int smip_ctx_create(int32_t device, smip_ctx** out)
+{
+    return guarded([&] {
+        require(out != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null output pointer");
+        int n = 0;
+        SMIP_HIP(hipGetDeviceCount(&n));
+        require(device >= 0 && device < n, SMIP_ERR_INVALID_ARGUMENT, "no such HIP device");
+        auto c = std::make_unique<smip_ctx>();
+        c->device = device;
+        SMIP_HIP(hipSetDevice(device));
+        SMIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+extern "C" SMIP_API int smip_ctx_destroy(smip_ctx* ctx)
+{
+    return guarded([&] {
+        if (!ctx)
+            return;
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+extern "C" SMIP_API int smip_ctx_synchronize(smip_ctx* ctx)
+{
+    return guarded([&] {
+        require(ctx != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null context");
+        ensure_device(ctx);
+        SMIP_HIP(hipDeviceSynchronize());
+    });
+}
+
+extern "C" SMIP_API int smip_ctx_mem_info(smip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    return guarded([&] {
+        require(ctx && free_bytes && total_bytes, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(ctx);
+        size_t f = 0, t = 0;
+        SMIP_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+        *total_bytes = t;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Bispectrum
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API int smip_bispec_create(smip_ctx* ctx, const uint64_t dims[4], int32_t dtype, smip_bispec** out)
+{
+    return guarded([&] {
+        require(ctx && dims && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(dtype == SMIP_DTYPE_C64 || dtype == SMIP_DTYPE_C128, SMIP_ERR_INVALID_ARGUMENT,
+            "bispectrum dtype must be complex64 or complex128");
+        ensure_device(ctx);
+        auto b = std::make_unique<smip_bispec>();
+        b->ctx = ctx;
+        b->dtype = dtype;
+        if (dims[0] || dims[1] || dims[2] || dims[3])
+            b->d = make_desc(dims[0], dims[1], dims[2], dims[3]);
+        else
+            b->d = Desc {}; // default-constructed: every descriptor field zero
+        if (dims[0] == 0 && dims[1] == 0 && dims[2] == 0 && dims[3] == 0)
+            b->d.base_size = 0;
+        if (b->d.base_size) {
+            b->data = dev_alloc<uint8_t>(b->d.base_size * b->esz());
+            SMIP_HIP(hipMemsetAsync(b->data, 0, b->d.base_size * b->esz(), ctx->stream));
+            SMIP_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *out = b.release();
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_destroy(smip_bispec* b)
+{
+    return guarded([&] {
+        if (!b)
+            return;
+        require(!b->borrowed, SMIP_ERR_LOGIC, "a job's bispectrum is owned by the job");
+        delete b;
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_info_get(const smip_bispec* b, smip_bispec_info* out)
+{
+    return guarded([&] {
+        require(b && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        for (int k = 0; k < 4; ++k) {
+            out->dims[k] = b->d.dims[k];
+            out->sizes[k] = b->d.sizes[k];
+            out->base_sizes[k] = b->d.base_sizes[k];
+            out->min_indices[k] = b->d.min_idx[k];
+            out->max_indices[k] = b->d.max_idx[k];
+        }
+        out->base_size = b->d.base_size;
+        out->totalsize = b->d.totalsize;
+        out->dtype = b->dtype;
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_accumulate_from_fft(smip_bispec* b, const double* fft, uint64_t ncols,
+    uint64_t nrows)
+{
+    return guarded([&] {
+        require(b && fft, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(ncols > 0 && nrows > 0 && ncols <= 65536 && nrows <= 65536, SMIP_ERR_INVALID_ARGUMENT,
+            "fft extents out of range");
+        if (b->d.base_size == 0)
+            return; // the loops of an empty bispectrum are empty
+        ensure_device(b->ctx);
+        hipStream_t s = b->ctx->stream;
+        const int32_t X = (int32_t)ncols, Y = (int32_t)nrows;
+        const HalfPlane hp = make_half_plane(X, Y, 0, 0);
+        auto full = dmake<double2>(ncols * nrows);
+        auto H = dmake<double2>((size_t)hp.fs);
+        SMIP_HIP(hipMemcpyAsync(full.get(), fft, sizeof(double2) * ncols * nrows, hipMemcpyHostToDevice, s));
+        const int64_t n = (int64_t)hp.hx * Y;
+        hipLaunchKernelGGL(k_scatter_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, full.get(), X, Y, hp,
+            H.get());
+        SMIP_HIP(hipGetLastError());
+        const LoopBounds L = loop_bounds(b->d, X, Y);
+        if (b->dtype == SMIP_DTYPE_C128)
+            launch_accum_plain_f64(b->d, L, hp, H.get(), static_cast<double2*>(b->data), s);
+        else
+            launch_accum_plain_f32(b->d, L, hp, H.get(), static_cast<float2*>(b->data), s);
+        SMIP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_get_elements(const smip_bispec* b, const int32_t* indices, uint64_t n,
+    double* values)
+{
+    return guarded([&] {
+        require(b && (n == 0 || (indices && values)), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (n == 0)
+            return;
+        ensure_device(b->ctx);
+        hipStream_t s = b->ctx->stream;
+        if (b->d.base_size == 0) {
+            // every lookup of an empty bispectrum ends at addr >= base_size (0)
+            fail(SMIP_ERR_OUT_OF_RANGE, "Bispectrum: element address out of bounds.");
+        }
+        auto idx = dmake<int32_t>(4 * n);
+        auto out = dmake<double2>(n);
+        auto st = dmake<int32_t>(n);
+        SMIP_HIP(hipMemcpyAsync(idx.get(), indices, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, s));
+        launch_get_elements(b->d, b->data, b->dtype == SMIP_DTYPE_C64, idx.get(), (int64_t)n, out.get(), st.get(),
+            s);
+        std::vector<int32_t> hst(n);
+        SMIP_HIP(hipMemcpyAsync(values, out.get(), sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipMemcpyAsync(hst.data(), st.get(), sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < n; ++i)
+            if (hst[i]) {
+                const int32_t* q = indices + 4 * i;
+                fail(hst[i], "Bispectrum: element access out of bounds: indices = [" + std::to_string(q[0]) + ", "
+                        + std::to_string(q[1]) + ", " + std::to_string(q[2]) + ", " + std::to_string(q[3]) + "]");
+            }
+    });
+}
+
+extern "C" SMIP_API uint64_t smip_bispec_calc_offset(const smip_bispec* b, const int32_t idx[4])
+{
+    if (!b || !idx)
+        return 0;
+    return calc_offset(b->d, idx[0], idx[1], idx[2], idx[3]);
+}
+
+extern "C" SMIP_API int smip_bispec_calc_indices(const smip_bispec* b, uint64_t addr, int32_t idx[4])
+{
+    return guarded([&] {
+        require(b && idx, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(b->d.base_size > 0 && addr < b->d.base_size, SMIP_ERR_OUT_OF_RANGE, "address out of range");
+        // include/bispectrum.h:280-316
+        const Desc& d = b->d;
+        uint64_t temp = d.base_size / d.base_sizes[0], rest = addr;
+        const uint64_t a0 = rest / temp;
+        rest -= temp * a0;
+        temp = d.base_sizes[2] * d.base_sizes[3];
+        const uint64_t a1 = rest / temp;
+        rest -= temp * a1;
+        temp = d.base_sizes[3];
+        const uint64_t a2 = rest / temp;
+        rest -= temp * a2;
+        int32_t v[4] = { (int32_t)a0, (int32_t)a1, (int32_t)a2, (int32_t)rest };
+        const int sign[4] = { -1, 1, -1, 1 };
+        for (int k = 0; k < 4; ++k) {
+            if (v[k] > (int32_t)d.sizes[k] / 2)
+                v[k] = (int32_t)((uint64_t)(int64_t)v[k] - d.sizes[k]);
+            idx[k] = v[k] * sign[k];
+        }
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_put_element(smip_bispec* b, const int32_t idx[4], const double value[2])
+{
+    return guarded([&] {
+        require(b && idx && value, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        const uint64_t addr = calc_offset(b->d, idx[0], idx[1], idx[2], idx[3]);
+        if (addr >= b->d.base_size)
+            fail(SMIP_ERR_OUT_OF_RANGE, "Bispectrum: put_element address out of bounds.");
+        ensure_device(b->ctx);
+        if (b->dtype == SMIP_DTYPE_C128) {
+            double2 v = make_double2(value[0], value[1]);
+            SMIP_HIP(hipMemcpy(static_cast<double2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice));
+        } else {
+            float2 v = make_float2((float)value[0], (float)value[1]);
+            SMIP_HIP(hipMemcpy(static_cast<float2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice));
+        }
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_download(const smip_bispec* b, void* host)
+{
+    return guarded([&] {
+        require(b && (host || b->d.base_size == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (!b->d.base_size)
+            return;
+        ensure_device(b->ctx);
+        SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
+        SMIP_HIP(hipMemcpy(host, b->data, b->d.base_size * b->esz(), hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_upload(smip_bispec* b, const void* host)
+{
+    return guarded([&] {
+        require(b && (host || b->d.base_size == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (!b->d.base_size)
+            return;
+        ensure_device(b->ctx);
+        SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
+        SMIP_HIP(hipMemcpy(b->data, host, b->d.base_size * b->esz(), hipMemcpyHostToDevice));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_write_file(const smip_bispec* b, const char* path)
+{
+    return guarded([&] {
+        require(b && path, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        std::vector<uint8_t> buf(b->d.base_size * b->esz());
+        if (b->d.base_size) {
+            ensure_device(b->ctx);
+            SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
+            SMIP_HIP(hipMemcpy(buf.data(), b->data, buf.size(), hipMemcpyDeviceToHost));
+        }
+        // header: size_t base_size, 4 x size_t constructor dims; then T[base_size]
+        FILE* f = std::fopen(path, "wb");
+        if (!f)
+            fail(SMIP_ERR_RUNTIME, std::string("unable to open file ") + path + " for writing");
+        const uint64_t hdr[5] = { b->d.base_size, b->d.dims[0], b->d.dims[1], b->d.dims[2], b->d.dims[3] };
+        std::fwrite(hdr, sizeof(uint64_t), 5, f);
+        if (!buf.empty())
+            std::fwrite(buf.data(), 1, buf.size(), f);
+        std::fclose(f);
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_read_file(smip_bispec* b, const char* path)
+{
+    return guarded([&] {
+        require(b && path, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        FILE* f = std::fopen(path, "rb");
+        if (!f)
+            fail(SMIP_ERR_RUNTIME, std::string("unable to open file ") + path);
+        uint64_t hdr[5];
+        if (std::fread(hdr, sizeof(uint64_t), 5, f) != 5) {
+            std::fclose(f);
+            fail(SMIP_ERR_RUNTIME, std::string("error reading bispectrum metadata from file ") + path);
+        }
+        if (b->borrowed) {
+            std::fclose(f);
+            fail(SMIP_ERR_LOGIC, "Cannot resize a reference-backed Array_base.");
+        }
+        const uint64_t size = hdr[0];
+        std::vector<uint8_t> buf(size * b->esz());
+        const size_t got = buf.empty() ? 0 : std::fread(buf.data(), b->esz(), size, f);
+        std::fclose(f);
+        ensure_device(b->ctx);
+        // Array_base::resize(size), then the descriptor of the stored dims
+        if (size != b->d.base_size) {
+            if (b->data)
+                SMIP_HIP(hipFree(b->data));
+            b->data = size ? dev_alloc<uint8_t>(size * b->esz()) : nullptr;
+        }
+        const bool zero = !(hdr[1] || hdr[2] || hdr[3] || hdr[4]);
+        b->d = zero ? Desc {} : make_desc(hdr[1], hdr[2], hdr[3], hdr[4]);
+        b->d.base_size = size;
+        if (got != size)
+            fail(SMIP_ERR_RUNTIME, std::string("error reading data block from file ") + path);
+        if (size)
+            SMIP_HIP(hipMemcpy(b->data, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_scalar_op(smip_bispec* b, int32_t op, double re, double im)
+{
+    return guarded([&] {
+        require(b != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(op >= SMIP_OP_ADD && op <= SMIP_OP_DIV, SMIP_ERR_INVALID_ARGUMENT, "unknown operator");
+        const bool f32 = b->dtype == SMIP_DTYPE_C64;
+        const bool zero = f32 ? ((float)re == 0.f && (float)im == 0.f) : (re == 0.0 && im == 0.0);
+        if (op == SMIP_OP_DIV && zero)
+            fail(SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
+        ensure_device(b->ctx);
+        if (f32)
+            launch_scale_c32(static_cast<float2*>(b->data), b->d.base_size, (float)re, (float)im, op,
+                b->ctx->stream);
+        else
+            launch_scale_c64(static_cast<double2*>(b->data), b->d.base_size, re, im, op, b->ctx->stream);
+        SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_array_op(smip_bispec* b, const smip_bispec* o, int32_t op)
+{
+    return guarded([&] {
+        require(b && o, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(op >= SMIP_OP_ADD && op <= SMIP_OP_DIV, SMIP_ERR_INVALID_ARGUMENT, "unknown operator");
+        if (std::memcmp(b->d.dims, o->d.dims, sizeof(b->d.dims)) != 0)
+            fail(SMIP_ERR_INVALID_ARGUMENT, "Bispectrum::operator+=(const Bispectrum) : operand dimension size mismatch");
+        require(b->dtype == o->dtype, SMIP_ERR_INVALID_ARGUMENT, "bispectrum element types differ");
+        if (!b->d.base_size)
+            return;
+        ensure_device(b->ctx);
+        const uint64_t n = b->d.base_size;
+        dim3 g((unsigned)((n + 255) / 256));
+        if (b->dtype == SMIP_DTYPE_C128)
+            hipLaunchKernelGGL(k_array_op<double2>, g, dim3(256), 0, b->ctx->stream, static_cast<double2*>(b->data),
+                static_cast<const double2*>(o->data), n, op);
+        else
+            hipLaunchKernelGGL(k_array_op<float2>, g, dim3(256), 0, b->ctx->stream, static_cast<float2*>(b->data),
+                static_cast<const float2*>(o->data), n, op);
+        SMIP_HIP(hipGetLastError());
+        SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* src)
+{
+    return guarded([&] {
+        require(dst && src, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(dst->dtype == src->dtype, SMIP_ERR_INVALID_ARGUMENT, "bispectrum element types differ");
+        if (dst == src)
+            return;
+        ensure_device(dst->ctx);
+        if (dst->d.base_size != src->d.base_size) {
+            require(!dst->borrowed, SMIP_ERR_LOGIC, "Cannot resize a reference-backed Array_base.");
+            if (dst->data)
+                SMIP_HIP(hipFree(dst->data));
+            dst->data = src->d.base_size ? dev_alloc<uint8_t>(src->d.base_size * src->esz()) : nullptr;
+        }
+        dst->d = src->d;
+        if (src->d.base_size) {
+            SMIP_HIP(hipStreamSynchronize(src->ctx->stream));
+            SMIP_HIP(hipMemcpy(dst->data, src->data, src->d.base_size * src->esz(), hipMemcpyDeviceToDevice));
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// phase reconstruction
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API void smip_next_reco_index(double* r, double* phi, int32_t* i, int32_t* j)
+{
+    if (r && phi && i && j)
+        next_reco_index(*r, *phi, *i, *j);
+}
+
+static void reconstruct_into(smip_ctx* ctx, hipStream_t s, const Desc& d, const void* B, int is_f32, uint64_t xs,
+    uint64_t ys, double reco_radius, double2* ph, uint8_t* fl, double* co, uint64_t* calls)
+{
+    check_reco_geometry(d, xs, ys);
+    (void)ctx;
+    const std::vector<int32_t> v = visit_list(reco_radius, (int64_t)xs, (int64_t)ys);
+    std::unique_ptr<RecoWork, void (*)(RecoWork*)> w(
+        reco_work_create(d, (int32_t)xs, (int32_t)ys, v.data(), (int64_t)(v.size() / 2)), reco_work_destroy);
+    const int c = run_reconstruct(w.get(), B, is_f32, ph, fl, co, s);
+    if (calls)
+        *calls = (uint64_t)c;
+}
+
+extern "C" SMIP_API int smip_reconstruct_phases(const smip_bispec* b, uint64_t xsize, uint64_t ysize,
+    double reco_radius, double* phases, uint8_t* flags, double* consistency)
+{
+    return guarded([&] {
+        require(b != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(b->ctx);
+        hipStream_t s = b->ctx->stream;
+        const uint64_t n = xsize * ysize;
+        require(b->d.base_size > 0 || reco_radius < 1.0, SMIP_ERR_OUT_OF_RANGE,
+            "Bispectrum: element address out of bounds.");
+        auto ph = dmake<double2>(n);
+        auto fl = dmake<uint8_t>(n);
+        auto co = dmake<double>(n);
+        reconstruct_into(b->ctx, s, b->d, b->data, b->dtype == SMIP_DTYPE_C64, xsize, ysize, reco_radius, ph.get(),
+            fl.get(), co.get(), nullptr);
+        if (phases)
+            SMIP_HIP(hipMemcpyAsync(phases, ph.get(), sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        if (flags)
+            SMIP_HIP(hipMemcpyAsync(flags, fl.get(), n, hipMemcpyDeviceToHost, s));
+        if (consistency)
+            SMIP_HIP(hipMemcpyAsync(consistency, co.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// CrossCorrelation<double>
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API int smip_xcorr_create(smip_ctx* ctx, const double* ref, uint64_t ncols, uint64_t nrows,
+    smip_xcorr** out)
+{
+    return guarded([&] {
+        require(ctx && ref && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(ctx);
+        auto x = std::make_unique<smip_xcorr>();
+        x->ctx = ctx;
+        x->X = (int)ncols;
+        x->Y = (int)nrows;
+        x->plan = fft_plan_create((int32_t)ncols, (int32_t)nrows);
+        const uint64_t np = ncols * nrows, nh = (ncols / 2 + 1) * nrows;
+        x->ref = dmake<double>(np);
+        x->frame = dmake<double>(np);
+        x->corr = dmake<double>(np);
+        x->ref_spec = dmake<double2>(nh);
+        x->spec = dmake<double2>(nh);
+        x->work = dmake<double2>(nh);
+        x->shift_dev = dmake<int32_t>(2);
+        hipStream_t s = ctx->stream;
+        SMIP_HIP(hipMemcpyAsync(x->ref.get(), ref, sizeof(double) * np, hipMemcpyHostToDevice, s));
+        launch_rfft_f64(x->plan, x->ref.get(), SMIP_DTYPE_F64, 1, false, x->ref_spec.get(), x->work.get(), s);
+        SMIP_HIP(hipStreamSynchronize(s));
+        *out = x.release();
+    });
+}
+
+extern "C" SMIP_API int smip_xcorr_destroy(smip_xcorr* x)
+{
+    return guarded([&] { delete x; });
+}
+
+extern "C" SMIP_API int smip_xcorr_correlate(smip_xcorr* x, const double* frame, uint64_t ncols, uint64_t nrows)
+{
+    return guarded([&] {
+        require(x && frame, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if ((int)ncols != x->X || (int)nrows != x->Y)
+            fail(SMIP_ERR_INVALID_ARGUMENT, "Matrix dimensions must match for correlation");
+        ensure_device(x->ctx);
+        hipStream_t s = x->ctx->stream;
+        SMIP_HIP(hipMemcpyAsync(x->frame.get(), frame, sizeof(double) * ncols * nrows, hipMemcpyHostToDevice, s));
+        launch_rfft_f64(x->plan, x->frame.get(), SMIP_DTYPE_F64, 1, false, x->spec.get(), x->work.get(), s);
+        launch_xcorr_f64(x->plan, x->ref_spec.get(), x->spec.get(), 1, x->corr.get(), x->work.get(), s);
+        SMIP_HIP(hipStreamSynchronize(s));
+        x->readiness = 1;
+    });
+}
+
+extern "C" SMIP_API int smip_xcorr_get_correlation(smip_xcorr* x, double* out)
+{
+    return guarded([&] {
+        require(x && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (x->readiness < 1)
+            fail(SMIP_ERR_BAD_FUNCTION_CALL, "bad_function_call");
+        ensure_device(x->ctx);
+        SMIP_HIP(hipMemcpy(out, x->corr.get(), sizeof(double) * x->X * x->Y, hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" SMIP_API int smip_xcorr_get_displacement(smip_xcorr* x, int32_t shift[2])
+{
+    return guarded([&] {
+        require(x && shift, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (x->readiness < 1)
+            fail(SMIP_ERR_BAD_FUNCTION_CALL, "bad_function_call");
+        if (x->readiness == 1) {
+            ensure_device(x->ctx);
+            hipStream_t s = x->ctx->stream;
+            launch_argmax_shift(x->corr.get(), 1, x->X, x->Y, x->shift_dev.get(), s);
+            SMIP_HIP(hipMemcpyAsync(x->shift, x->shift_dev.get(), sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
+            SMIP_HIP(hipStreamSynchronize(s));
+            x->readiness = 2;
+        }
+        shift[0] = x->shift[0];
+        shift[1] = x->shift[1];
+    });
+}
+
+extern "C" SMIP_API int smip_xcorr_displacement(smip_xcorr* x, const double* frame, uint64_t ncols, uint64_t nrows,
+    int32_t shift[2])
+{
+    if (x)
+        x->readiness = 0;
+    int rc = smip_xcorr_correlate(x, frame, ncols, nrows);
+    if (rc)
+        return rc;
+    return smip_xcorr_get_displacement(x, shift);
+}
+
+// ---------------------------------------------------------------------------
+// window
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API int smip_general_hamming(uint64_t xsize, uint64_t ysize, double aperture, double alpha,
+    double* out)
+{
+    return guarded([&] {
+        require(out || xsize * ysize == 0, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (!(aperture > 0.))
+            fail(SMIP_ERR_DOMAIN, "WindowFunction aperture size <= 0");
+        const int32_t X = (int32_t)xsize, Y = (int32_t)ysize;
+        const double trigarg = k2pi / aperture, c1 = alpha, c2 = 1. - alpha;
+        for (int32_t i = -(X >> 1); i <= -X / 2 + X - 1; ++i)
+            for (int32_t j = -(Y >> 1); j <= -Y / 2 + Y - 1; ++j) {
+                const double t1 = c1 + c2 * std::cos(trigarg * static_cast<double>(i));
+                const double t2 = c1 + c2 * std::cos(trigarg * static_cast<double>(j));
+                const int32_t col = i < 0 ? i + X : i, row = j < 0 ? j + Y : j;
+                out[(int64_t)row * X + col] = t1 * t2;
+            }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// frames
+// ---------------------------------------------------------------------------
+extern "C" SMIP_API int smip_frames_create(smip_ctx* ctx, int32_t dtype, uint64_t ncols, uint64_t nrows,
+    uint64_t count, smip_frames** out)
+{
+    return guarded([&] {
+        require(ctx && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(dtype >= SMIP_DTYPE_U8 && dtype <= SMIP_DTYPE_F64, SMIP_ERR_INVALID_ARGUMENT, "bad frame dtype");
+        ensure_device(ctx);
+        auto f = std::make_unique<smip_frames>();
+        f->ctx = ctx;
+        f->dtype = dtype;
+        f->X = ncols;
+        f->Y = nrows;
+        f->count = count;
+        f->data = dev_alloc<uint8_t>(count * ncols * nrows * dtype_size(dtype));
+        *out = f.release();
+    });
+}
+
+extern "C" SMIP_API int smip_frames_destroy(smip_frames* f)
+{
+    return guarded([&] { delete f; });
+}
+
+extern "C" SMIP_API int smip_frames_upload(smip_frames* f, uint64_t first, uint64_t count, const void* host)
+{
+    return guarded([&] {
+        require(f && host, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(first + count <= f->count, SMIP_ERR_OUT_OF_RANGE, "frame range outside the stack");
+        ensure_device(f->ctx);
+        const size_t fb = f->X * f->Y * dtype_size(f->dtype);
+        SMIP_HIP(hipMemcpy(static_cast<uint8_t*>(f->data) + first * fb, host, count * fb, hipMemcpyHostToDevice));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+// ---------------------------------------------------------------------------
+namespace {
+
+struct JobSizes {
+    Desc d;
+    LoopBounds L;
+    HalfPlane hp;
+    int vy_tile;
+    int batch;
+    uint64_t bytes;
+};
+
+JobSizes job_sizes(const smip_job_params& p)
+{
+    auto pow2 = [](uint64_t n) { return n >= 8 && n <= 2048 && (n & (n - 1)) == 0; };
+    require(pow2(p.ncols) && pow2(p.nrows), SMIP_ERR_INVALID_ARGUMENT,
+        "job frame extents must be powers of two in [8, 2048]");
+    require(p.bdepth >= 1 && p.bdepth / 2 < std::min(p.ncols, p.nrows) / 2, SMIP_ERR_INVALID_ARGUMENT,
+        "bispectrum depth must satisfy 1 <= b and b/2 < N/2");
+    require(p.frame_dtype == SMIP_DTYPE_U8 || p.frame_dtype == SMIP_DTYPE_U16 || p.frame_dtype == SMIP_DTYPE_F32,
+        SMIP_ERR_INVALID_ARGUMENT, "frame dtype must be u8, u16 or f32");
+    require(p.precision == SMIP_PRECISION_FP64 || p.precision == SMIP_PRECISION_MIXED, SMIP_ERR_INVALID_ARGUMENT,
+        "unknown precision mode");
+    require(p.frames_per_batch >= 0 && p.frames_per_batch <= 4096, SMIP_ERR_INVALID_ARGUMENT,
+        "frames_per_batch out of range");
+    JobSizes z;
+    z.d = make_desc(p.ncols, p.nrows, p.bdepth, p.bdepth);
+    z.L = loop_bounds(z.d, (int64_t)p.ncols, (int64_t)p.nrows);
+    const bool mixed = p.precision == SMIP_PRECISION_MIXED;
+    z.vy_tile = mixed ? (p.vy_tile > 0 ? p.vy_tile : 0) : 0;
+    if (mixed && z.vy_tile == 0) {
+        // same choice as tile_plan_create: fewest wasted vy lanes
+        const int nvy = z.L.max4 - z.L.min4 + 1;
+        int best = 9, bw = 1 << 30;
+        for (int c : { 9, 13, 17, 25, 33 }) {
+            const int w = ((nvy + c - 1) / c) * c - nvy;
+            if (w < bw || (w == bw && c > best)) {
+                best = c;
+                bw = w;
+            }
+        }
+        z.vy_tile = best;
+    }
+    const int pad = mixed ? tile_plan_pad_needed(z.d, z.vy_tile) : 0;
+    const int tail = mixed ? tile_plan_tail_needed(z.d, (int)p.nrows, z.vy_tile) : 0;
+    z.hp = make_half_plane((int32_t)p.ncols, (int32_t)p.nrows, pad, tail);
+    z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (mixed ? 256 : 16);
+    const uint64_t X = p.ncols, Y = p.nrows, hx = X / 2 + 1, np = X * Y;
+    uint64_t b = z.d.base_size * sizeof(double2) + hx * Y * sizeof(double);
+    if (mixed)
+        b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(float2) + hx * Y * sizeof(float2));
+    else
+        b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
+    if (p.saa)
+        b += hx * Y * sizeof(double2) + (uint64_t)z.batch * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
+            + np * sizeof(double);
+    b += (uint64_t)z.batch * np * dtype_size(p.frame_dtype);             // host upload staging
+    b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
+    z.bytes = b;
+    return z;
+}
+
+void record(smip_job* j, int kind, hipEvent_t a, hipEvent_t b) { j->events.push_back({ a, b, kind }); }
+
+hipEvent_t mark(smip_job* j)
+{
+    hipEvent_t e;
+    SMIP_HIP(hipEventCreate(&e));
+    SMIP_HIP(hipEventRecord(e, j->stream));
+    return e;
+}
+
+void drain_events(smip_job* j)
+{
+    for (auto& e : j->events) {
+        float ms = 0.f;
+        SMIP_HIP(hipEventSynchronize(e.b));
+        SMIP_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+        j->ms[e.kind] += ms;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    j->events.clear();
+}
+
+void job_process(smip_job* j, const void* frames, uint64_t count)
+{
+    const smip_job_params& p = j->p;
+    const size_t fb = p.ncols * p.nrows * dtype_size(p.frame_dtype);
+    const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
+    hipStream_t s = j->stream;
+    const bool mixed = p.precision == SMIP_PRECISION_MIXED;
+    for (uint64_t off = 0; off < count;) {
+        const int64_t nf = (int64_t)std::min<uint64_t>((uint64_t)j->batch, count - off);
+        const uint8_t* fr = static_cast<const uint8_t*>(frames) + off * fb;
+        hipEvent_t e0 = mark(j);
+        if (mixed) {
+            launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H32.get(), j->work32.get(),
+                j->P_half.get(), s);
+            launch_accum_tile(j->tile, j->hp, j->H32.get(), nf, j->B64.get(), s);
+        } else {
+            launch_forward_f64(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H64.get(), j->work64.get(),
+                j->P_half.get(), s);
+            for (int64_t f = 0; f < nf; ++f) // one frame per launch: the plain drop-in
+                launch_accum_plain_f64(j->d, j->L, j->hp, j->H64.get() + f * j->hp.fs, j->B64.get(), s);
+        }
+        hipEvent_t e1 = mark(j);
+        record(j, 0, e0, e1);
+        if (p.saa) {
+            hipEvent_t e2 = mark(j);
+            int64_t first = 0;
+            if (j->nframes == 0) {
+                // sumarray = Re(frame 0); the reference spectrum is frame 0's r2c
+                const int64_t np = (int64_t)X * Y;
+                hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, fr,
+                    p.frame_dtype, np, j->sum.get());
+                launch_rfft_f64(j->plan, fr, p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), s);
+                first = 1;
+            }
+            const int64_t m = nf - first;
+            if (m > 0) {
+                const uint8_t* fm = fr + first * fb;
+                launch_rfft_f64(j->plan, fm, p.frame_dtype, m, true, j->sspec.get(), j->swork.get(), s);
+                launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), m, j->corr.get(), j->swork.get(), s);
+                int32_t* sh = j->shifts.get() + 2 * (j->nframes + first);
+                launch_argmax_shift(j->corr.get(), m, X, Y, sh, s);
+                launch_saa_add(fm, p.frame_dtype, m, X, Y, sh, j->sum.get(), s);
+            }
+            hipEvent_t e3 = mark(j);
+            record(j, 1, e2, e3);
+        }
+        j->nframes += (uint64_t)nf;
+        off += (uint64_t)nf;
+    }
+}
+
+void grow_shifts(smip_job* j, uint64_t total)
+{
+    if (!j->p.saa || total <= j->shifts_cap)
+        return;
+    uint64_t cap = std::max<uint64_t>(total, 2 * j->shifts_cap);
+    auto n = dmake<int32_t>(2 * cap);
+    SMIP_HIP(hipMemsetAsync(n.get(), 0, sizeof(int32_t) * 2 * cap, j->stream));
+    if (j->shifts_cap)
+        SMIP_HIP(hipMemcpyAsync(n.get(), j->shifts.get(), sizeof(int32_t) * 2 * j->shifts_cap,
+            hipMemcpyDeviceToDevice, j->stream));
+    SMIP_HIP(hipStreamSynchronize(j->stream));
+    j->shifts = std::move(n);
+    j->shifts_cap = cap;
+}
+
+} // namespace
+
+extern "C" SMIP_API int smip_job_footprint(const smip_job_params* p, uint64_t* bytes)
+{
+    return guarded([&] {
+        require(p && bytes, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        *bytes = job_sizes(*p).bytes;
+    });
+}
+
+extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p, smip_job** out)
+{
+    return guarded([&] {
+        require(ctx && p && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(ctx);
+        const JobSizes z = job_sizes(*p);
+        size_t fr = 0, tot = 0;
+        SMIP_HIP(hipMemGetInfo(&fr, &tot));
+        if (z.bytes > fr)
+            fail(SMIP_ERR_NO_MEMORY, "job footprint " + std::to_string(z.bytes) + " B exceeds free device memory");
+        auto j = std::make_unique<smip_job>();
+        j->ctx = ctx;
+        j->p = *p;
+        j->d = z.d;
+        j->L = z.L;
+        j->hp = z.hp;
+        j->batch = z.batch;
+        j->footprint = z.bytes;
+        SMIP_HIP(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+        hipStream_t s = j->stream;
+        const uint64_t X = p->ncols, Y = p->nrows, hx = X / 2 + 1, np = X * Y;
+        j->plan = fft_plan_create((int32_t)X, (int32_t)Y);
+        j->B64 = dmake<double2>(z.d.base_size);
+        SMIP_HIP(hipMemsetAsync(j->B64.get(), 0, sizeof(double2) * z.d.base_size, s));
+        j->P_half = dmake<double>(hx * Y);
+        SMIP_HIP(hipMemsetAsync(j->P_half.get(), 0, sizeof(double) * hx * Y, s));
+        if (p->precision == SMIP_PRECISION_MIXED) {
+            j->tile = tile_plan_create(z.d, z.L, z.hp, z.vy_tile);
+            j->H32 = dmake<float2>((size_t)j->batch * z.hp.fs);
+            SMIP_HIP(hipMemsetAsync(j->H32.get(), 0, sizeof(float2) * j->batch * z.hp.fs, s)); // zero padding rows
+            j->work32 = dmake<float2>((size_t)j->batch * hx * Y);
+        } else {
+            j->H64 = dmake<double2>((size_t)j->batch * z.hp.fs);
+            SMIP_HIP(hipMemsetAsync(j->H64.get(), 0, sizeof(double2) * j->batch * z.hp.fs, s));
+            j->work64 = dmake<double2>((size_t)j->batch * hx * Y);
+        }
+        if (p->saa) {
+            j->ref_spec = dmake<double2>(hx * Y);
+            j->sspec = dmake<double2>((size_t)j->batch * hx * Y);
+            j->swork = dmake<double2>((size_t)j->batch * hx * Y);
+            j->corr = dmake<double>((size_t)j->batch * np);
+            j->sum = dmake<double>(np);
+            SMIP_HIP(hipMemsetAsync(j->sum.get(), 0, sizeof(double) * np, s));
+        }
+        j->phases = dmake<double2>(np);
+        j->image = dmake<double2>(np);
+        j->flags = dmake<uint8_t>(np);
+        j->cons = dmake<double>(np);
+        j->scratch = dmake<double>(16);
+        SMIP_HIP(hipStreamSynchronize(s));
+        auto v = std::make_unique<smip_bispec>();
+        v->ctx = ctx;
+        v->d = z.d;
+        v->dtype = SMIP_DTYPE_C128;
+        v->data = j->B64.get();
+        v->borrowed = true;
+        j->view = std::move(v);
+        *out = j.release();
+    });
+}
+
+extern "C" SMIP_API int smip_job_destroy(smip_job* j)
+{
+    return guarded([&] {
+        if (!j)
+            return;
+        ensure_device(j->ctx);
+        (void)hipStreamSynchronize(j->stream);
+        delete j;
+    });
+}
+
+extern "C" SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, uint64_t first, uint64_t count)
+{
+    return guarded([&] {
+        require(j && f, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(f->dtype == j->p.frame_dtype && f->X == j->p.ncols && f->Y == j->p.nrows, SMIP_ERR_INVALID_ARGUMENT,
+            "frame stack does not match the job");
+        require(first + count <= f->count, SMIP_ERR_OUT_OF_RANGE, "frame range outside the stack");
+        ensure_device(j->ctx);
+        grow_shifts(j, j->nframes + count);
+        const size_t fb = f->X * f->Y * dtype_size(f->dtype);
+        job_process(j, static_cast<const uint8_t*>(f->data) + first * fb, count);
+    });
+}
+
+extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t count)
+{
+    return guarded([&] {
+        require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        ensure_device(j->ctx);
+        grow_shifts(j, j->nframes + count);
+        const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
+        if (!j->upload)
+            j->upload = dmake<uint8_t>((size_t)j->batch * fb);
+        for (uint64_t off = 0; off < count;) {
+            const uint64_t nf = std::min<uint64_t>((uint64_t)j->batch, count - off);
+            SMIP_HIP(hipStreamSynchronize(j->stream)); // staging is reused
+            SMIP_HIP(hipMemcpyAsync(j->upload.get(), static_cast<const uint8_t*>(frames) + off * fb, nf * fb,
+                hipMemcpyHostToDevice, j->stream));
+            job_process(j, j->upload.get(), nf);
+            off += nf;
+        }
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_job_finalize(smip_job* j)
+{
+    return guarded([&] {
+        require(j != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(j->nframes > 0, SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
+        ensure_device(j->ctx);
+        hipStream_t s = j->stream;
+        const smip_job_params& p = j->p;
+        // bispectrum /= complex(n, 0)
+        launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, s);
+        hipEvent_t e0 = mark(j);
+        uint64_t calls = 0;
+        reconstruct_into(j->ctx, s, j->d, j->B64.get(), 0, p.ncols, p.nrows, p.reco_radius, j->phases.get(),
+            j->flags.get(), j->cons.get(), &calls);
+        j->reco_calls = calls;
+        hipEvent_t e1 = mark(j);
+        record(j, 2, e0, e1);
+        hipEvent_t e2 = mark(j);
+        const double norm = (double)(j->nframes * p.ncols * p.nrows);
+        launch_synth_prepare(j->P_half.get(), j->phases.get(), (int32_t)p.ncols, (int32_t)p.nrows, p.reco_radius,
+            norm, j->image.get(), s);
+        launch_c2c_inverse_f64(j->plan, j->image.get(), nullptr, s);
+        launch_abs_max_normalize(j->image.get(), (int64_t)(p.ncols * p.nrows), j->scratch.get(), s);
+        hipEvent_t e3 = mark(j);
+        record(j, 3, e2, e3);
+        SMIP_HIP(hipStreamSynchronize(s));
+        j->finalized = true;
+    });
+}
+
+extern "C" SMIP_API int smip_job_synchronize(smip_job* j)
+{
+    return guarded([&] {
+        require(j != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(j->ctx);
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_stats(smip_job* j, smip_job_stats* out)
+{
+    return guarded([&] {
+        require(j && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(j->ctx);
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+        drain_events(j);
+        out->frames = j->nframes;
+        out->accumulate_ms = j->ms[0];
+        out->saa_ms = j->ms[1];
+        out->reco_ms = j->ms[2];
+        out->synth_ms = j->ms[3];
+        out->footprint = j->footprint;
+        out->reco_calls = j->reco_calls;
+        out->cells_per_frame = j->tile ? tile_plan_cells(j->tile) : 0;
+    });
+}
+
+extern "C" SMIP_API int smip_job_bispectrum(smip_job* j, smip_bispec** out)
+{
+    return guarded([&] {
+        require(j && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        ensure_device(j->ctx);
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+        *out = j->view.get();
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_powerspec(smip_job* j, double* out)
+{
+    return guarded([&] {
+        require(j && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(j->nframes > 0, SMIP_ERR_RUNTIME, "no frames accumulated");
+        ensure_device(j->ctx);
+        const uint64_t np = j->p.ncols * j->p.nrows;
+        auto tmp = dmake<double>(np);
+        launch_power_expand(j->P_half.get(), (int32_t)j->p.ncols, (int32_t)j->p.nrows, (double)(j->nframes * np),
+            tmp.get(), j->stream);
+        SMIP_HIP(hipMemcpyAsync(out, tmp.get(), sizeof(double) * np, hipMemcpyDeviceToHost, j->stream));
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_sum_image(smip_job* j, double* out)
+{
+    return guarded([&] {
+        require(j && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(j->p.saa, SMIP_ERR_LOGIC, "job was created without shift-and-add");
+        require(j->nframes > 0, SMIP_ERR_RUNTIME, "no frames accumulated");
+        ensure_device(j->ctx);
+        const uint64_t np = j->p.ncols * j->p.nrows;
+        SMIP_HIP(hipMemcpyAsync(out, j->sum.get(), sizeof(double) * np, hipMemcpyDeviceToHost, j->stream));
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+        const double n = (double)j->nframes; // sumarray /= nframes
+        for (uint64_t k = 0; k < np; ++k)
+            out[k] /= n;
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_shifts(smip_job* j, int32_t* out, uint64_t count)
+{
+    return guarded([&] {
+        require(j && (out || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(j->p.saa, SMIP_ERR_LOGIC, "job was created without shift-and-add");
+        require(count <= j->nframes, SMIP_ERR_OUT_OF_RANGE, "more shifts requested than frames");
+        ensure_device(j->ctx);
+        if (count)
+            SMIP_HIP(hipMemcpyAsync(out, j->shifts.get(), sizeof(int32_t) * 2 * count, hipMemcpyDeviceToHost,
+                j->stream));
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_phases(smip_job* j, double* phases, uint8_t* flags, double* consistency)
+{
+    return guarded([&] {
+        require(j != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(j->finalized, SMIP_ERR_BAD_FUNCTION_CALL, "job not finalized");
+        ensure_device(j->ctx);
+        const uint64_t np = j->p.ncols * j->p.nrows;
+        hipStream_t s = j->stream;
+        if (phases)
+            SMIP_HIP(hipMemcpyAsync(phases, j->phases.get(), sizeof(double2) * np, hipMemcpyDeviceToHost, s));
+        if (flags)
+            SMIP_HIP(hipMemcpyAsync(flags, j->flags.get(), np, hipMemcpyDeviceToHost, s));
+        if (consistency)
+            SMIP_HIP(hipMemcpyAsync(consistency, j->cons.get(), sizeof(double) * np, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+extern "C" SMIP_API int smip_job_get_image(smip_job* j, double* out)
+{
+    return guarded([&] {
+        require(j && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(j->finalized, SMIP_ERR_BAD_FUNCTION_CALL, "job not finalized");
+        ensure_device(j->ctx);
+        const uint64_t np = j->p.ncols * j->p.nrows;
+        SMIP_HIP(hipMemcpyAsync(out, j->image.get(), sizeof(double2) * np, hipMemcpyDeviceToHost, j->stream));
+        SMIP_HIP(hipStreamSynchronize(j->stream));
+    });
+}

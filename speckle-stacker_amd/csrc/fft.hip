@@ -1,0 +1,525 @@
+// fft.hip -- batched 2-D FFTs for frame stacks on gfx950.
+//
+// Replaces the FFTW calls of the reference path:
+//   forward c2c of each frame            smip-cli/main.cpp:256-259,271,296
+//   r2c / c2r of the cross-correlation   include/crosscorrel.h:112-141
+//   backward c2c of the synthesis        smip-cli/main.cpp:340-346
+// Conventions are FFTW's: forward exp(-2 pi i k x / n), backward exp(+...),
+// both unnormalised.
+//
+// Each 1-D transform runs in LDS as a Stockham autosort sequence of radix-4
+// stages with a final radix-2 stage when log2(n) is odd; one workgroup holds
+// several transforms so every stage keeps all 256 lanes busy.  Real frames are
+// transformed two rows at a time (z = row0 + i row1) and split afterwards.
+// Twiddles come from tables computed on the host in double precision.
+#include <cmath>
+#include <vector>
+
+#include "smip_complex.h"
+#include "smip_internal.h"
+
+namespace smip_amd {
+
+struct FftPlan {
+    int32_t X, Y, lx, ly;
+    float2* twx32 = nullptr;
+    float2* twy32 = nullptr;
+    double2* twx64 = nullptr;
+    double2* twy64 = nullptr;
+};
+
+static int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n)
+        ++l;
+    return l;
+}
+
+template <typename V>
+static V* upload_twiddles(int n)
+{
+    std::vector<V> h(n);
+    const double tp = 6.283185307179586476925286766559;
+    for (int k = 0; k < n; ++k) {
+        // exp(-2 pi i k / n); the angle is reduced to the first half turn so the
+        // table is symmetric to the last bit
+        double c = std::cos(tp * (double)k / (double)n);
+        double s = std::sin(tp * (double)k / (double)n);
+        h[k].x = (typename scalar_of<V>::type)c;
+        h[k].y = (typename scalar_of<V>::type)(-s);
+    }
+    V* d = nullptr;
+    SMIP_HIP(hipMalloc(&d, sizeof(V) * n));
+    SMIP_HIP(hipMemcpy(d, h.data(), sizeof(V) * n, hipMemcpyHostToDevice));
+    return d;
+}
+
+FftPlan* fft_plan_create(int32_t ncols, int32_t nrows)
+{
+    auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
+    if (!pow2(ncols) || !pow2(nrows) || ncols > 2048 || nrows > 2048)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "frame extents must be powers of two in [8, 2048]");
+    auto* p = new FftPlan;
+    p->X = ncols;
+    p->Y = nrows;
+    p->lx = ilog2(ncols);
+    p->ly = ilog2(nrows);
+    try {
+        p->twx32 = upload_twiddles<float2>(ncols);
+        p->twy32 = upload_twiddles<float2>(nrows);
+        p->twx64 = upload_twiddles<double2>(ncols);
+        p->twy64 = upload_twiddles<double2>(nrows);
+    } catch (...) {
+        fft_plan_destroy(p);
+        throw;
+    }
+    return p;
+}
+
+void fft_plan_destroy(FftPlan* p)
+{
+    if (!p)
+        return;
+    (void)hipFree(p->twx32);
+    (void)hipFree(p->twy32);
+    (void)hipFree(p->twx64);
+    (void)hipFree(p->twy64);
+    delete p;
+}
+
+template <typename V>
+static const V* tw_x(const FftPlan* p);
+template <>
+const float2* tw_x<float2>(const FftPlan* p) { return p->twx32; }
+template <>
+const double2* tw_x<double2>(const FftPlan* p) { return p->twx64; }
+template <typename V>
+static const V* tw_y(const FftPlan* p);
+template <>
+const float2* tw_y<float2>(const FftPlan* p) { return p->twy32; }
+template <>
+const double2* tw_y<double2>(const FftPlan* p) { return p->twy64; }
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT of nb transforms of length n = 2^log2n stored back to back.
+// Stage with radix R and span Ns: butterfly j reads x[j + r n/R], applies the
+// twiddles w^(r k n/(Ns R)) with k = j mod Ns, and writes the R-point DFT to
+// y[(j - k) R + k + r Ns].  Returns the buffer holding the result.
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ V* lds_fft(V* a, V* b, int n, int log2n, int nb, const V* __restrict__ tw, bool inv)
+{
+    const int nthreads = blockDim.x;
+    const int tid = threadIdx.x;
+    int Ns = 1;
+    const int q4 = n >> 2, lq4 = log2n - 2;
+    for (int s = 0; s < (log2n >> 1); ++s) {
+        const int ltw = log2n - 2 - 2 * s; // log2(n / (4 Ns))
+        for (int t = tid; t < nb * q4; t += nthreads) {
+            const int f = t >> lq4;
+            const int j = t & (q4 - 1);
+            const V* src = a + f * n;
+            V* dst = b + f * n;
+            const int k = j & (Ns - 1);
+            V v0 = src[j], v1 = src[j + q4], v2 = src[j + 2 * q4], v3 = src[j + 3 * q4];
+            if (Ns > 1) {
+                const int e = k << ltw;
+                V w1 = tw[e], w2 = tw[2 * e], w3 = tw[3 * e];
+                if (inv) {
+                    w1 = cconj(w1);
+                    w2 = cconj(w2);
+                    w3 = cconj(w3);
+                }
+                v1 = cmul(v1, w1);
+                v2 = cmul(v2, w2);
+                v3 = cmul(v3, w3);
+            }
+            V a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+            a3 = inv ? cmul_pi(a3) : cmul_mi(a3);
+            const int d = ((j - k) << 2) + k;
+            dst[d] = cadd(a0, a2);
+            dst[d + Ns] = cadd(a1, a3);
+            dst[d + 2 * Ns] = csub(a0, a2);
+            dst[d + 3 * Ns] = csub(a1, a3);
+        }
+        __syncthreads();
+        V* t = a;
+        a = b;
+        b = t;
+        Ns <<= 2;
+    }
+    if (log2n & 1) {
+        const int q2 = n >> 1, lq2 = log2n - 1;
+        for (int t = tid; t < nb * q2; t += nthreads) {
+            const int f = t >> lq2;
+            const int j = t & (q2 - 1);
+            const V* src = a + f * n;
+            V* dst = b + f * n;
+            const int k = j & (Ns - 1);
+            V v0 = src[j], v1 = src[j + q2];
+            if (Ns > 1) {
+                V w = tw[k];
+                if (inv)
+                    w = cconj(w);
+                v1 = cmul(v1, w);
+            }
+            const int d = ((j - k) << 1) + k;
+            dst[d] = cadd(v0, v1);
+            dst[d + Ns] = csub(v0, v1);
+        }
+        __syncthreads();
+        V* t = a;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+template <typename S, typename In>
+__device__ inline S load_sample(const In* p, int64_t i, bool absval)
+{
+    S v = (S)p[i];
+    return absval ? (v < (S)0 ? -v : v) : v;
+}
+
+// LDS budget per workgroup (two ping-pong buffers): 64 KiB.
+constexpr int kLdsBytes = 65536;
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Row pass: real rows y0 = 2p, y1 = 2p+1 of frame f -> G[f][y][kx], kx in [0, X/2]
+// ---------------------------------------------------------------------------
+template <typename V, typename In>
+__global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ frames, int X, int Y, int lx, int ppw,
+    const V* __restrict__ tw, V* __restrict__ G, int hx, int absval)
+{
+    using S = typename scalar_of<V>::type;
+    extern __shared__ char smem[];
+    V* A = reinterpret_cast<V*>(smem);
+    V* B = A + ppw * X;
+    const int64_t f = blockIdx.y;
+    const int pair0 = blockIdx.x * ppw;
+    const In* fr = frames + f * (int64_t)X * Y;
+    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
+        const int p = t >> lx, x = t & (X - 1);
+        const int64_t y0 = 2 * (pair0 + p);
+        A[t] = mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
+    }
+    __syncthreads();
+    const V* R = lds_fft<V>(A, B, X, lx, ppw, tw, false);
+    const S half = (S)0.5;
+    for (int t = threadIdx.x; t < ppw * hx; t += blockDim.x) {
+        const int p = t / hx, k = t - p * hx;
+        const V zk = R[p * X + k];
+        const V zm = cconj(R[p * X + ((X - k) & (X - 1))]);
+        const V s = cadd(zk, zm), dlt = cmul_mi(csub(zk, zm));
+        const int64_t y0 = 2 * (pair0 + p);
+        G[(f * Y + y0) * hx + k] = mk<V>(s.x * half, s.y * half);
+        G[(f * Y + y0 + 1) * hx + k] = mk<V>(dlt.x * half, dlt.y * half);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: G[f][y][kx] -> transform along y.
+//   MODE 0: half-plane layout H[f][X/2 - kx][pad + r] = conj(R[(Y/2 - r) mod Y][kx])
+//           (the spectrum at ux = -kx <= 0, uy = r - Y/2 of the real frame)
+//   MODE 1: natural r2c layout R[f][ky][kx]
+// ---------------------------------------------------------------------------
+template <typename V, int MODE, typename Out>
+__global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, int X, int Y, int ly, int hx, int cb,
+    const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp)
+{
+    extern __shared__ char smem[];
+    V* A = reinterpret_cast<V*>(smem);
+    V* B = A + cb * Y;
+    const int64_t f = blockIdx.y;
+    const int kx0 = blockIdx.x * cb;
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        const int kx = kx0 + c;
+        A[c * Y + y] = kx < hx ? G[(f * Y + y) * hx + kx] : mk<V>(0, 0);
+    }
+    __syncthreads();
+    const V* R = lds_fft<V>(A, B, Y, ly, cb, tw, false);
+    if (MODE == 0) {
+        for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+            const int c = t / Y, r = t - c * Y;
+            const int kx = kx0 + c;
+            if (kx >= hx)
+                continue;
+            const V v = R[c * Y + ((Y / 2 - r) & (Y - 1))];
+            Out o;
+            o.x = v.x;
+            o.y = -v.y;
+            out[f * hp.fs + (int64_t)(X / 2 - kx) * hp.rs + hp.pad + r] = o;
+        }
+    } else {
+        for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+            const int ky = t / cb, c = t - ky * cb;
+            const int kx = kx0 + c;
+            if (kx >= hx)
+                continue;
+            const V v = R[c * Y + ky];
+            Out o;
+            o.x = v.x;
+            o.y = v.y;
+            out[(f * Y + ky) * hx + kx] = o;
+        }
+    }
+}
+
+// P[c][r] += |H[f][c][pad + r]|^2 for f = 0..nf-1 in order (FP64), the power
+// spectrum of smip-cli/main.cpp:274-279,300-305 on the stored half plane.
+template <typename V>
+__global__ void k_power_acc(const V* __restrict__ H, int64_t nf, HalfPlane hp, double* __restrict__ P)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)hp.hx * hp.nrows;
+    if (i >= n)
+        return;
+    const int c = (int)(i / hp.nrows), r = (int)(i - (int64_t)c * hp.nrows);
+    const V* p = H + (int64_t)c * hp.rs + hp.pad + r;
+    double acc = P[i];
+    for (int64_t f = 0; f < nf; ++f) {
+        const V v = p[f * hp.fs];
+        const double re = (double)v.x, im = (double)v.y;
+        acc += re * re + im * im;
+    }
+    P[i] = acc;
+}
+
+// Column pass of the cross-correlation: C = conj(Rref) * R along ky, inverse
+// transform, G[f][y][kx].  include/crosscorrel.h:126-141.
+__global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __restrict__ Rref,
+    const double2* __restrict__ R, int X, int Y, int ly, int hx, int cb, const double2* __restrict__ tw,
+    double2* __restrict__ G)
+{
+    extern __shared__ char smem[];
+    double2* A = reinterpret_cast<double2*>(smem);
+    double2* B = A + cb * Y;
+    const int64_t f = blockIdx.y;
+    const int kx0 = blockIdx.x * cb;
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        const int kx = kx0 + c;
+        double2 v = mk<double2>(0, 0);
+        if (kx < hx) {
+            const double2 a = Rref[(int64_t)y * hx + kx];
+            const double2 b = R[(f * Y + y) * hx + kx];
+            // std::conj(a) * b with the (ac - bd, ad + bc) operand order
+            v = cmul(cconj(a), b);
+        }
+        A[c * Y + y] = v;
+    }
+    __syncthreads();
+    const double2* Rs = lds_fft<double2>(A, B, Y, ly, cb, tw, true);
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        const int kx = kx0 + c;
+        if (kx < hx)
+            G[(f * Y + y) * hx + kx] = Rs[c * Y + y];
+    }
+}
+
+// Row pass of a c2r transform, two rows at a time: z = IDFT(H0 + i H1) gives
+// row0 = Re z, row1 = Im z.  The imaginary parts of the kx = 0 and kx = X/2 terms
+// are dropped, as a c2r transform of Hermitian data does.
+__global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict__ G, int X, int Y, int lx, int ppw,
+    int hx, const double2* __restrict__ tw, double* __restrict__ out)
+{
+    extern __shared__ char smem[];
+    double2* A = reinterpret_cast<double2*>(smem);
+    double2* B = A + ppw * X;
+    const int64_t f = blockIdx.y;
+    const int pair0 = blockIdx.x * ppw;
+    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
+        const int p = t >> lx, k = t & (X - 1);
+        const int64_t y0 = 2 * (pair0 + p);
+        const double2* g0 = G + (f * Y + y0) * hx;
+        const double2* g1 = g0 + hx;
+        double2 h0, h1;
+        if (k <= X / 2) {
+            h0 = g0[k];
+            h1 = g1[k];
+            if (k == 0 || k == X / 2) {
+                h0.y = 0.0;
+                h1.y = 0.0;
+            }
+        } else {
+            h0 = cconj(g0[X - k]);
+            h1 = cconj(g1[X - k]);
+        }
+        A[t] = cadd(h0, cmul_pi(h1));
+    }
+    __syncthreads();
+    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
+    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
+        const int p = t >> lx, x = t & (X - 1);
+        const int64_t y0 = 2 * (pair0 + p);
+        out[(f * Y + y0) * X + x] = Z[t].x;
+        out[(f * Y + y0 + 1) * X + x] = Z[t].y;
+    }
+}
+
+// Complex 2-D transform passes (in place), used by the synthesis.
+__global__ __launch_bounds__(kThreads) void k_rows_c2c(double2* __restrict__ data, int X, int lx, int rpw,
+    const double2* __restrict__ tw, int inv)
+{
+    extern __shared__ char smem[];
+    double2* A = reinterpret_cast<double2*>(smem);
+    double2* B = A + rpw * X;
+    double2* base = data + (int64_t)blockIdx.x * rpw * X;
+    for (int t = threadIdx.x; t < rpw * X; t += blockDim.x)
+        A[t] = base[t];
+    __syncthreads();
+    const double2* Z = lds_fft<double2>(A, B, X, lx, rpw, tw, inv != 0);
+    for (int t = threadIdx.x; t < rpw * X; t += blockDim.x)
+        base[t] = Z[t];
+}
+
+__global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ data, int X, int Y, int ly, int cb,
+    const double2* __restrict__ tw, int inv)
+{
+    extern __shared__ char smem[];
+    double2* A = reinterpret_cast<double2*>(smem);
+    double2* B = A + cb * Y;
+    const int x0 = blockIdx.x * cb;
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        A[c * Y + y] = (x0 + c < X) ? data[(int64_t)y * X + x0 + c] : mk<double2>(0, 0);
+    }
+    __syncthreads();
+    const double2* Z = lds_fft<double2>(A, B, Y, ly, cb, tw, inv != 0);
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        if (x0 + c < X)
+            data[(int64_t)y * X + x0 + c] = Z[c * Y + y];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <typename V>
+static int per_wg(int n, int cap)
+{
+    const int elems = kLdsBytes / (2 * (int)sizeof(V));
+    int k = elems / n;
+    if (k < 1)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "transform length exceeds the LDS budget");
+    if (k > cap)
+        k = cap;
+    // keep a power of two so every block gets the same count
+    int p = 1;
+    while (p * 2 <= k)
+        p *= 2;
+    return p;
+}
+
+template <typename V>
+static void launch_rows_fwd(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval, V* G,
+    hipStream_t s)
+{
+    const int X = p->X, Y = p->Y, hx = X / 2 + 1;
+    const int ppw = per_wg<V>(X, Y / 2);
+    dim3 grid((Y / 2) / ppw, (unsigned)nf);
+    const size_t lds = 2 * sizeof(V) * ppw * X;
+    switch (dtype) {
+    case SMIP_DTYPE_U8:
+        hipLaunchKernelGGL((k_rows_fwd<V, uint8_t>), grid, dim3(kThreads), lds, s, (const uint8_t*)frames, X, Y, p->lx,
+            ppw, tw_x<V>(p), G, hx, (int)absval);
+        break;
+    case SMIP_DTYPE_U16:
+        hipLaunchKernelGGL((k_rows_fwd<V, uint16_t>), grid, dim3(kThreads), lds, s, (const uint16_t*)frames, X, Y,
+            p->lx, ppw, tw_x<V>(p), G, hx, (int)absval);
+        break;
+    case SMIP_DTYPE_F32:
+        hipLaunchKernelGGL((k_rows_fwd<V, float>), grid, dim3(kThreads), lds, s, (const float*)frames, X, Y, p->lx,
+            ppw, tw_x<V>(p), G, hx, (int)absval);
+        break;
+    case SMIP_DTYPE_F64:
+        hipLaunchKernelGGL((k_rows_fwd<V, double>), grid, dim3(kThreads), lds, s, (const double*)frames, X, Y, p->lx,
+            ppw, tw_x<V>(p), G, hx, (int)absval);
+        break;
+    default:
+        fail(SMIP_ERR_INVALID_ARGUMENT, "unsupported frame dtype");
+    }
+    SMIP_HIP(hipGetLastError());
+}
+
+template <typename V, int MODE, typename Out>
+static void launch_cols_fwd(const FftPlan* p, const V* G, int64_t nf, Out* out, const HalfPlane& hp, hipStream_t s)
+{
+    const int X = p->X, Y = p->Y, hx = X / 2 + 1;
+    const int cb = per_wg<V>(Y, 64);
+    dim3 grid((hx + cb - 1) / cb, (unsigned)nf);
+    const size_t lds = 2 * sizeof(V) * cb * Y;
+    hipLaunchKernelGGL((k_cols_fwd<V, MODE, Out>), grid, dim3(kThreads), lds, s, G, X, Y, p->ly, hx, cb, tw_y<V>(p),
+        out, hp);
+    SMIP_HIP(hipGetLastError());
+}
+
+template <typename V>
+static void launch_power(const V* H, int64_t nf, const HalfPlane& hp, double* P, hipStream_t s)
+{
+    const int64_t n = (int64_t)hp.hx * hp.nrows;
+    hipLaunchKernelGGL((k_power_acc<V>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, nf, hp, P);
+    SMIP_HIP(hipGetLastError());
+}
+
+void launch_forward_f32(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const HalfPlane& hp, float2* out, float2* work, double* power_acc, hipStream_t s)
+{
+    launch_rows_fwd<float2>(p, frames, dtype, nf, absval, work, s);
+    launch_cols_fwd<float2, 0, float2>(p, work, nf, out, hp, s);
+    if (power_acc)
+        launch_power<float2>(out, nf, hp, power_acc, s);
+}
+
+void launch_forward_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const HalfPlane& hp, double2* out, double2* work, double* power_acc, hipStream_t s)
+{
+    launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
+    launch_cols_fwd<double2, 0, double2>(p, work, nf, out, hp, s);
+    if (power_acc)
+        launch_power<double2>(out, nf, hp, power_acc, s);
+}
+
+void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval, double2* out,
+    double2* work, hipStream_t s)
+{
+    HalfPlane hp {};
+    launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
+    launch_cols_fwd<double2, 1, double2>(p, work, nf, out, hp, s);
+}
+
+void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf, double* corr,
+    double2* work, hipStream_t s)
+{
+    const int X = p->X, Y = p->Y, hx = X / 2 + 1;
+    const int cb = per_wg<double2>(Y, 64);
+    hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
+        2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+    SMIP_HIP(hipGetLastError());
+    const int ppw = per_wg<double2>(X, Y / 2);
+    hipLaunchKernelGGL(k_rows_c2r, dim3((Y / 2) / ppw, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
+        s, work, X, Y, p->lx, ppw, hx, p->twx64, corr);
+    SMIP_HIP(hipGetLastError());
+}
+
+void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipStream_t s)
+{
+    (void)work;
+    const int X = p->X, Y = p->Y;
+    const int rpw = per_wg<double2>(X, Y);
+    hipLaunchKernelGGL(k_rows_c2c, dim3(Y / rpw), dim3(kThreads), 2 * sizeof(double2) * rpw * X, s, data, X, p->lx,
+        rpw, p->twx64, 1);
+    SMIP_HIP(hipGetLastError());
+    const int cb = per_wg<double2>(Y, 64);
+    hipLaunchKernelGGL(k_cols_c2c, dim3((X + cb - 1) / cb), dim3(kThreads), 2 * sizeof(double2) * cb * Y, s, data, X,
+        Y, p->ly, cb, p->twy64, 1);
+    SMIP_HIP(hipGetLastError());
+}
+
+} // namespace smip_amd

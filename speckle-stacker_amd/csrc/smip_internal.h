@@ -1,0 +1,218 @@
+// smip_internal.h -- shared definitions of the MI355X speckle-masking engine
+// (host launchers, device helpers, error plumbing).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/smip_mi355x.h"
+
+namespace smip_amd {
+
+// ---------------------------------------------------------------------------
+// errors: C++ exceptions inside the library, mapped to status codes at the ABI
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m)
+        : std::runtime_error(m)
+        , code(c)
+    {
+    }
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        fail(SMIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define SMIP_HIP(x) ::smip_amd::hip_check((x), #x)
+
+// ---------------------------------------------------------------------------
+// Bispectrum descriptor (include/bispectrum.h:104-112,143-160,203-220)
+// ---------------------------------------------------------------------------
+struct Desc {
+    uint64_t dims[4];
+    uint64_t sizes[4];
+    uint64_t base_sizes[4];
+    uint64_t base_size;
+    uint64_t totalsize;
+    int32_t min_idx[4];
+    int32_t max_idx[4];
+};
+
+Desc make_desc(uint64_t ncols, uint64_t nrows, uint64_t b0, uint64_t b1);
+
+// Reference-layout address arithmetic, shared by host and device.
+// calc_offset keeps the size_t (mod 2^64) arithmetic and the absence of a range
+// check of include/bispectrum.h:324-349.
+__host__ __device__ inline uint64_t calc_offset(const Desc& d, int32_t i0, int32_t i1, int32_t i2, int32_t i3)
+{
+    int32_t a[4] = { -i0, i1, -i2, i3 };
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (a[k] < 0)
+            a[k] = (int32_t)((uint64_t)(int64_t)a[k] + d.sizes[k]);
+    uint64_t addr = (uint64_t)(int64_t)a[0];
+    addr = addr * d.base_sizes[1] + (uint64_t)(int64_t)a[1];
+    addr = addr * d.base_sizes[2] + (uint64_t)(int64_t)a[2];
+    addr = addr * d.base_sizes[3] + (uint64_t)(int64_t)a[3];
+    return addr;
+}
+
+__host__ __device__ inline int32_t iabs32(int32_t x) { return x < 0 ? -x : x; }
+
+// get_element's address resolution: pre-swap, range check, symmetry
+// canonicalisation (T1/T7/T6/T9/T3/T12), post-swap, calc_offset, address check
+// (include/bispectrum.h:162-174,482-512,553-589).  Returns 0 or SMIP_ERR_OUT_OF_RANGE.
+__host__ __device__ inline int resolve_element(const Desc& d, int32_t ux, int32_t uy, int32_t vx, int32_t vy,
+    uint64_t* addr, bool* conj)
+{
+    if (iabs32(vx) > d.max_idx[2] || iabs32(vy) > d.max_idx[3]) {
+        int32_t t = ux;
+        ux = vx;
+        vx = t;
+        t = uy;
+        uy = vy;
+        vy = t;
+    }
+    if (ux < d.min_idx[0] || ux > d.max_idx[0] || uy < d.min_idx[1] || uy > d.max_idx[1] || vx < d.min_idx[2]
+        || vx > d.max_idx[2] || vy < d.min_idx[3] || vy > d.max_idx[3])
+        return SMIP_ERR_OUT_OF_RANGE;
+    int32_t a0, a1, a2, a3;
+    bool c = false;
+    if (ux <= 0 && vx <= 0) { // T1
+        a0 = ux, a1 = uy, a2 = vx, a3 = vy;
+    } else if (ux > 0 && vx > 0) { // T7
+        c = true;
+        a0 = -ux, a1 = -uy, a2 = -vx, a3 = -vy;
+    } else if (ux > 0) { // vx <= 0
+        if (ux + vx > 0) { // T6
+            a0 = -ux - vx, a1 = -uy - vy, a2 = vx, a3 = vy;
+        } else { // T9
+            c = true;
+            a0 = -ux, a1 = -uy, a2 = vx + ux, a3 = vy + uy;
+        }
+    } else { // ux <= 0, vx > 0
+        if (ux + vx > 0) { // T3
+            a0 = ux, a1 = uy, a2 = -vx - ux, a3 = -vy - uy;
+        } else { // T12
+            c = true;
+            a0 = ux + vx, a1 = uy + vy, a2 = -vx, a3 = -vy;
+        }
+    }
+    if (iabs32(a2) > d.max_idx[2] || iabs32(a3) > d.max_idx[3]) {
+        int32_t t = a0;
+        a0 = a2;
+        a2 = t;
+        t = a1;
+        a1 = a3;
+        a3 = t;
+    }
+    uint64_t off = calc_offset(d, a0, a1, a2, a3);
+    if (off >= d.base_size)
+        return SMIP_ERR_OUT_OF_RANGE;
+    *addr = off;
+    *conj = c;
+    return 0;
+}
+
+// Loop bounds of accumulate_from_fft (include/bispectrum.h:383-391) for an FFT
+// array of ncols x nrows.
+struct LoopBounds {
+    int32_t min1, min2, min3, min4, max1, max2, max4;
+};
+LoopBounds loop_bounds(const Desc& d, int64_t ncols, int64_t nrows);
+
+// ---------------------------------------------------------------------------
+// Device spectrum layout ("half plane", x <= 0):
+//   H[frame][c][PAD + r], c = ux + (ncols>>1) in [0, ncols>>1],
+//   r = uy + (nrows>>1) in [0, nrows), zero rows outside [PAD, PAD + nrows).
+// Row stride RS complex elements, frame stride HX * RS.
+// ---------------------------------------------------------------------------
+struct HalfPlane {
+    int32_t ncols, nrows; // frame extents
+    int32_t hx;           // number of stored columns = (ncols >> 1) + 1
+    int32_t pad;          // zero rows before row 0
+    int32_t rs;           // row stride (complex elements)
+    int64_t fs;           // frame stride = hx * rs
+};
+HalfPlane make_half_plane(int32_t ncols, int32_t nrows, int32_t pad, int32_t tail);
+
+// ---------------------------------------------------------------------------
+// launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+enum class FrameType : int { u8 = SMIP_DTYPE_U8, u16 = SMIP_DTYPE_U16, f32 = SMIP_DTYPE_F32, f64 = SMIP_DTYPE_F64 };
+size_t dtype_size(int dtype);
+
+struct FftPlan; // twiddle tables for one frame shape
+FftPlan* fft_plan_create(int32_t ncols, int32_t nrows);
+void fft_plan_destroy(FftPlan*);
+
+// frames -> half-plane spectrum (float2 or double2), optional |x| of the input
+// and optional FP64 power-spectrum accumulation over the nf frames.
+// work must hold nf * hx * nrows complex values of the spectrum precision.
+void launch_forward_f32(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const HalfPlane& hp, float2* out, float2* work, double* power_acc, hipStream_t s);
+void launch_forward_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const HalfPlane& hp, double2* out, double2* work, double* power_acc, hipStream_t s);
+// natural r2c spectrum R[f][ky][kx] (kx in [0, ncols/2]) in FP64 for the
+// cross-correlation; work as above.
+void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    double2* out, double2* work, hipStream_t s);
+// c = c2r(conj(R_ref) * R_f) for nf frames; out nf * nrows * ncols doubles.
+void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
+    double* corr, double2* work, hipStream_t s);
+// unnormalised backward complex 2-D FFT, FP64, in place on nrows x ncols.
+void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipStream_t s);
+
+// bispectrum accumulation
+void launch_accum_plain_f64(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, const double2* H,
+    double2* B, hipStream_t s);
+void launch_accum_plain_f32(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, const double2* H,
+    float2* B, hipStream_t s);
+
+struct TilePlan; // work-item table of the batched accumulate
+TilePlan* tile_plan_create(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, int vy_tile);
+void tile_plan_destroy(TilePlan*);
+int tile_plan_vy_tile(const TilePlan*);
+int tile_plan_pad_needed(const Desc& d, int vy_tile);
+int tile_plan_tail_needed(const Desc& d, int nrows, int vy_tile);
+void launch_accum_tile(const TilePlan* tp, const HalfPlane& hp, const float2* H, int64_t nf, double2* B64,
+    hipStream_t s);
+uint64_t tile_plan_cells(const TilePlan* tp); // cells computed per frame
+
+void launch_scale_c64(double2* B, uint64_t n, double re, double im, int op, hipStream_t s);
+void launch_scale_c32(float2* B, uint64_t n, float re, float im, int op, hipStream_t s);
+void launch_convert_c64_to_c32(const double2* in, float2* out, uint64_t n, hipStream_t s);
+void launch_get_elements(const Desc& d, const void* B, int is_f32, const int32_t* idx, int64_t n, double2* out,
+    int32_t* status, hipStream_t s);
+
+// shift-and-add
+void launch_argmax_shift(const double* corr, int64_t nf, int32_t ncols, int32_t nrows, int32_t* shifts,
+    hipStream_t s);
+void launch_saa_add(const void* frames, int dtype, int64_t nf, int32_t ncols, int32_t nrows,
+    const int32_t* shifts, double* sum, hipStream_t s);
+
+// synthesis
+void launch_synth_prepare(const double* power_half, const double2* phases, int32_t ncols, int32_t nrows,
+    double reco_radius, double norm, double2* out, hipStream_t s);
+void launch_abs_max_normalize(double2* data, int64_t n, double* scratch, hipStream_t s);
+void launch_power_expand(const double* power_half, int32_t ncols, int32_t nrows, double norm, double* out,
+    hipStream_t s);
+
+// phase reconstruction (device); visits = in-range spiral visits (x,y) pairs
+struct RecoWork;
+RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host,
+    int64_t nvisits);
+void reco_work_destroy(RecoWork*);
+int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,
+    hipStream_t s);
+
+} // namespace smip_amd

@@ -1,0 +1,181 @@
+"""GPU parity of the HIP engine against the CPU oracle (through the C ABI).
+
+Bars (stated per test):
+  * integer / index work -- shifts, flags, element addresses: bit-exact;
+  * the drop-in accumulate_from_fft on identical FFT input: bit-exact (both sides
+    evaluate the reference's operation order without FMA contraction);
+  * FP64 engine (FP64 FFT of our own, FP64 triple products): relative 1e-11;
+  * mixed engine (FP32 FFT and triple products per batch, FP64 fold): bispectrum
+    relative error 2e-5 of max|B|, image relative L2 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import smip_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(n, N, dtype=np.uint8, seed=1, hi=255):
+    rng = np.random.default_rng(seed)
+    # a bright blob pair on a noisy background keeps the correlation peak unique
+    y, x = np.mgrid[0:N, 0:N]
+    out = np.empty((n, N, N), dtype=dtype)
+    for f in range(n):
+        cx, cy = rng.integers(N // 4, 3 * N // 4, size=2)
+        img = 0.6 * hi * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / 8.0)
+        img += 0.35 * hi * np.exp(-((x - cx - 5) ** 2 + (y - cy - 3) ** 2) / 6.0)
+        img += rng.uniform(0, 0.05 * hi, size=(N, N))
+        out[f] = np.clip(np.rint(img), 0, hi).astype(dtype) if np.issubdtype(dtype, np.integer) else img.astype(dtype)
+    return out
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("shape,b", [((16, 16), 8), ((12, 10), 6), ((9, 7), 5), ((32, 32), 12)])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_accumulate_from_fft_bitexact(ctx, shape, b, dtype):
+    """Bispectrum<T>::accumulate_from_fft, include/bispectrum.h:356-409, any frame shape."""
+    rng = np.random.default_rng(7)
+    nrows, ncols = shape
+    d = O.desc(ncols, nrows, b)
+    ref = O.new_bispectrum(d, dtype)
+    gpu = S.Bispectrum(ctx, (ncols, nrows, b, b), dtype)
+    for _ in range(3):
+        fft = O.fft2(rng.normal(size=shape) * 10)
+        O.accumulate_from_fft(d, ref, fft)
+        gpu.accumulate_from_fft(fft)
+    got = gpu.download()
+    assert got.dtype == ref.dtype
+    assert np.array_equal(got, ref)
+
+
+def test_get_elements_bitexact_with_aliasing(ctx):
+    """get_element with all six symmetry cases and calc_offset's wrapping lookups."""
+    rng = np.random.default_rng(3)
+    N, b = 32, 12
+    d = O.desc(N, N, b)
+    B = (rng.normal(size=int(d.base_size)) + 1j * rng.normal(size=int(d.base_size))).astype(np.complex128)
+    gpu = S.Bispectrum(ctx, (N, N, b, b))
+    gpu.upload(B)
+    h = b // 2
+    idx = []
+    for _ in range(4000):  # the lookups calc_phase makes: u anywhere, v in the stored v range
+        ux, uy = rng.integers(-N // 2, N // 2, size=2)
+        vx, vy = rng.integers(-h, h + 1, size=2)
+        idx.append((ux, uy, vx, vy))
+    ok = [i for i in idx]
+    want = []
+    aliased = 0
+    for i in ok:
+        want.append(O.get_element(d, B, i))
+        aliased += O.lookup(d, i)[2]
+    got = gpu.get_elements(ok)
+    assert aliased > 0
+    assert np.array_equal(got, np.array(want))
+    # an out-of-range lookup raises std::out_of_range's code
+    with pytest.raises(S.SmipError) as e:
+        gpu.get_element((N, 0, 0, 0))
+    assert e.value.code == S.ERR_OUT_OF_RANGE
+
+
+@pytest.mark.parametrize("N,b,p", [(32, 8, 16), (64, 16, 32)])
+def test_reconstruct_phases_dropin(ctx, N, b, p):
+    """reconstruct_phases on the same bispectrum: flags bit-exact, phases to 1e-9."""
+    frames = _frames(6, N).astype(np.float64)
+    B, _ = O.power_and_bispectrum(frames, b)
+    d = O.desc(N, N, b)
+    ref = O.reconstruct_phases(d, B, N, N, p)
+    gpu = S.Bispectrum(ctx, (N, N, b, b))
+    gpu.upload(B)
+    ph, fl, co = gpu.reconstruct_phases(N, N, p)
+    assert np.array_equal(fl, ref.flags)
+    m = ref.flags.astype(bool)
+    assert np.max(np.abs(ph[m] - ref.phases[m])) < 1e-9
+    assert np.max(np.abs(co - ref.consistency)) < 1e-9
+    assert np.all(ph[~m] == 0)
+
+
+def test_reconstruct_phases_c64_bispectrum(ctx):
+    """The CLI's Bispectrum<complex<float>> input (smip-cli/main.cpp:319)."""
+    N, b, p = 32, 8, 16
+    frames = _frames(5, N).astype(np.float64)
+    B, _ = O.power_and_bispectrum(frames, b, bispec_f32=True)
+    d = O.desc(N, N, b)
+    ref = O.reconstruct_phases(d, B, N, N, p)
+    gpu = S.Bispectrum(ctx, (N, N, b, b), np.complex64)
+    gpu.upload(B)
+    ph, fl, co = gpu.reconstruct_phases(N, N, p)
+    assert np.array_equal(fl, ref.flags)
+    assert np.max(np.abs(ph - ref.phases)) < 1e-9
+
+
+def test_xcorr_displacement(ctx):
+    """CrossCorrelation<double>: correlation array and exact integer shifts."""
+    rng = np.random.default_rng(11)
+    N = 64
+    ref = _frames(1, N, np.float64, seed=5)[0]
+    xc = S.CrossCorrelation(ctx, ref)
+    with pytest.raises(S.SmipError) as e:
+        xc.get_displacement()
+    assert e.value.code == S.ERR_BAD_FUNCTION_CALL
+    for _ in range(12):
+        dx, dy = (int(v) for v in rng.integers(-20, 21, size=2))
+        fr = np.roll(ref, (dy, dx), axis=(0, 1))
+        want = O.displacement(O.xcorr(ref, fr))
+        assert xc(fr) == want == (dx, dy)
+    xc.correlate(fr)
+    c = xc.get_correlation_array()
+    assert _rel(c, O.xcorr(ref, fr)) < 1e-12
+    with pytest.raises(S.SmipError) as e:
+        xc.correlate(np.zeros((N, N // 2)))
+    assert e.value.code == S.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("N,b,p,n", [(32, 8, 16, 9), (64, 16, 32, 20)])
+def test_engine_fp64(ctx, N, b, p, n):
+    """The plain FP64 engine against the FP64 oracle pipeline."""
+    frames = _frames(n, N, seed=N)
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=S.PRECISION_FP64, frames_per_batch=4)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert _rel(got.bispectrum, want.bispectrum) < 1e-11
+    assert _rel(got.powerspec, want.powerspec) < 1e-11
+    assert np.array_equal(got.flags, want.flags)
+    assert np.max(np.abs(got.phases - want.phases)) < 1e-9
+    assert _rel_l2(got.image, want.image) < 1e-9
+
+
+@pytest.mark.parametrize("N,b,p,n,batch", [(32, 8, 16, 9, 4), (64, 16, 32, 20, 8), (64, 12, 30, 17, 0)])
+def test_engine_mixed(ctx, N, b, p, n, batch):
+    """The batched mixed-precision engine against the FP64 oracle pipeline."""
+    frames = _frames(n, N, seed=N + 1)
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=S.PRECISION_MIXED, frames_per_batch=batch)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert _rel(got.bispectrum, want.bispectrum) < 2e-5
+    # cells the reference never writes stay exactly zero
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    assert _rel(got.powerspec, want.powerspec) < 1e-5
+    assert np.array_equal(got.flags, want.flags)
+    assert _rel_l2(got.image, want.image) < 1e-5
+
+
+@pytest.mark.parametrize("vy_tile", [9, 13, 17, 25, 33])
+def test_tile_shapes(ctx, vy_tile):
+    """Every instantiated register-tile shape gives the same bispectrum."""
+    N, b, n = 64, 24, 6
+    frames = _frames(n, N, seed=3)
+    B, _ = O.power_and_bispectrum(frames.astype(np.float64), b)
+    got = S.run_pipeline(ctx, frames, b, 20, precision=S.PRECISION_MIXED, saa=False, vy_tile=vy_tile)
+    assert _rel(got.bispectrum, B) < 2e-5
+    assert np.array_equal(got.bispectrum == 0, B == 0)
