@@ -1,0 +1,242 @@
+#!/usr/bin/env python
+"""Bispectrum frames/s of the MI355X speckle-masking engine.
+
+Workload (the metric's configuration): triple256 -- 2000 synthetic speckle
+frames of 256x256 uint16 (12-bit), triple-star object, pupil 64 px, D/r0 = 20,
+1e5 photons/frame; bispectrum depth b = 48, phase radius p = 128.
+
+A step is one pass of the hot path (FFT + |F|^2 + triple-product accumulation)
+over the whole 2000-frame stack, already resident in HBM, in the batched
+mixed-precision form (FP32 triple products per batch folded into the FP64
+bispectrum).  Besides the JSON contract fields the line carries the plain
+one-frame-at-a-time FP64 GPU path's frames/s and the speedup over it, the p50 /
+p99 latency of complete jobs (accumulate, shift-and-add, phase recovery,
+synthesis), the image error of the mixed path against the FP64 GPU path, the
+roofline position of the triple-product kernel and the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import threading
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "speckle-stacker_amd", "python"))
+
+import smip_amd as S  # noqa: E402
+
+VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (dense)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(path: str):
+    """HBM bytes per launch of the triple-product kernel from a committed
+    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE lines), if present."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(frames: np.ndarray, b: int, seconds: float) -> dict:
+    """The reference's per-frame CPU path (FP64 FFT, complex<float> accumulate,
+    smip-cli/main.cpp:256-305) as restated by the oracle, single thread."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # the checker, used here only as the CPU baseline
+
+    f64 = frames.astype(np.float64)
+    t0 = time.perf_counter()
+    O.power_and_bispectrum(f64[:1], b, bispec_f32=True, fast=True)
+    one = time.perf_counter() - t0
+    k = int(max(1, min(len(f64) - 1, seconds / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    O.power_and_bispectrum(f64[1:1 + k], b, bispec_f32=True, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{k} of the workload's frames through the oracle's FP64 FFT + complex<float> "
+                      f"accumulate (-O3 -ffast-math), 1 thread, {dt:.1f} s on {platform.processor() or 'host'}"}
+
+
+def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int):
+    """njobs complete reconstructions submitted together; each thread drives one job."""
+    lat = [0.0] * njobs
+    errs = []
+    stats = [None] * njobs
+    t_submit = time.perf_counter()
+
+    def worker(i):
+        try:
+            p = S.job_params(W["N"], W["N"], W["b"], W["p"], W["dtype"], S.PRECISION_MIXED, saa=True,
+                             frames_per_batch=batch, vy_tile=vy)
+            job = S.Job(ctx, p)
+            job.add_frames(dev)
+            job.finalize()
+            lat[i] = (time.perf_counter() - t_submit) * 1e3
+            stats[i] = job.stats()
+            job.close()
+        except Exception as e:  # reported, not hidden
+            errs.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(njobs)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    if errs:
+        raise RuntimeError("; ".join(errs))
+    return lat, stats
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="triple256")
+    ap.add_argument("--frames", type=int, default=0, help="override the workload's frame count")
+    ap.add_argument("--batch", type=int, default=0, help="frames per FP64 fold (0 = engine default)")
+    ap.add_argument("--vy-tile", type=int, default=0)
+    ap.add_argument("--jobs", type=int, default=8, help="concurrent jobs of the latency measurement")
+    ap.add_argument("--plain-frames", type=int, default=400)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-latency", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    W = dict(S.WORKLOADS[args.workload])
+    n = args.frames or W["n"]
+    N, b, pr = W["N"], W["b"], W["p"]
+    log(f"generating {n} frames of {N}x{N} {np.dtype(W['dtype']).name} ({args.workload})")
+    t0 = time.perf_counter()
+    frames = S.synth_frames(S.synth_params(args.workload), 0, n, threads=min(16, os.cpu_count() or 1))
+    log(f"generated in {time.perf_counter() - t0:.1f} s")
+
+    ctx = S.Context(0)
+    dev = S.Frames(ctx, frames)
+
+    # ---- batched mixed-precision accumulate: the metric ----------------------
+    p = S.job_params(N, N, b, pr, W["dtype"], S.PRECISION_MIXED, saa=False, frames_per_batch=args.batch,
+                     vy_tile=args.vy_tile)
+    job = S.Job(ctx, p)
+    for _ in range(args.warmup):
+        job.add_frames(dev)
+    ctx.synchronize()
+    st0 = job.stats()
+    log(f"timing {args.steps} steps of {n} frames (mixed)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        job.add_frames(dev)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    st1 = job.stats()
+    job.close()
+    elapsed = t1 - t0
+    fps = args.steps * n / elapsed
+    tile_ms = st1["tile_ms"] - st0["tile_ms"]
+    launches = st1["tile_launches"] - st0["tile_launches"]
+    acc_ms = st1["accumulate_ms"] - st0["accumulate_ms"]
+    frames_per_launch = args.steps * n / launches
+    triples = st1["triples_per_frame"]
+    flops_per_launch = 14.0 * triples * frames_per_launch
+    avg_launch_s = tile_ms / launches * 1e-3
+    achieved = flops_per_launch / avg_launch_s / 1e12
+    log(f"mixed: {fps:.0f} frames/s; triple-product kernel {achieved:.1f} TFLOP/s, "
+        f"{tile_ms / (args.steps * n) * 1e3:.2f} us/frame of {acc_ms / (args.steps * n) * 1e3:.2f} us/frame")
+
+    # ---- the plain FP64 GPU drop-in (one frame per accumulate launch) --------
+    n64 = min(n, args.plain_frames)
+    p64 = S.job_params(N, N, b, pr, W["dtype"], S.PRECISION_FP64, saa=False)
+    j64 = S.Job(ctx, p64)
+    j64.add_frames(dev, 0, min(16, n64))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    j64.add_frames(dev, 0, n64)
+    ctx.synchronize()
+    fps64 = n64 / (time.perf_counter() - t0)
+    j64.close()
+    log(f"plain FP64: {fps64:.0f} frames/s over {n64} frames; speedup {fps / fps64:.1f}x")
+
+    out = {
+        "metric": "bispectrum frames/s @256^2,b=48,p=128 vs plain FP64 GPU path; p50/p99 job latency",
+        "value": round(fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n} frames {N}x{N} {np.dtype(W['dtype']).name}, "
+                               f"triple star, b={b}, p={pr}, mixed precision (FP32 batch, FP64 fold)",
+                   "frames": n, "ncols": N, "nrows": N, "bdepth": b, "reco_radius": pr,
+                   "frames_per_batch": args.batch or 256, "triples_per_frame": triples},
+        "plain_fp64_frames_per_s": round(fps64, 1),
+        "speedup_vs_plain_fp64": round(fps / fps64, 2),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / VALU_FP32_PEAK_TFLOPS, 4),
+                     "traffic": traffic_from_profile(os.path.join(REPO, "profiles", "tile_triple256_pmc.txt")),
+                     "kernel": "k_accum_tile", "flops_per_launch": flops_per_launch,
+                     "avg_launch_ms": round(tile_ms / launches, 4)},
+    }
+
+    # ---- complete jobs: latency and accuracy ---------------------------------
+    if not args.no_latency:
+        log(f"running {args.jobs} concurrent complete jobs (accumulate + SAA + phases + synthesis)")
+        lat, stats = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile)
+        lat.sort()
+        out["job_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2),
+                                 "p99": round(float(np.percentile(lat, 99)), 2), "jobs": args.jobs,
+                                 "frames_per_job": n, "arrival": "all jobs submitted at t=0"}
+        s0 = stats[0]
+        out["job_stage_ms"] = {k: round(s0[k], 2) for k in ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}
+        log(f"latency p50 {out['job_latency_ms']['p50']} ms p99 {out['job_latency_ms']['p99']} ms; "
+            f"stages {out['job_stage_ms']}")
+        # accuracy of the mixed path against the FP64 GPU path on the same frames
+        na = min(n, 200)
+        rm = S.run_pipeline(ctx, frames[:na], b, pr, precision=S.PRECISION_MIXED, with_bispectrum=False)
+        rd = S.run_pipeline(ctx, frames[:na], b, pr, precision=S.PRECISION_FP64, with_bispectrum=False)
+        out["mixed_vs_fp64_gpu"] = {
+            "frames": na,
+            "image_rel_l2": float(np.linalg.norm(np.abs(rm.image) - np.abs(rd.image)) / np.linalg.norm(np.abs(rd.image))),
+            "flags_equal": bool(np.array_equal(rm.flags, rd.flags)),
+            "shifts_equal": bool(np.array_equal(rm.shifts, rd.shifts)),
+        }
+        log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
+
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(frames, b, args.cpu_seconds)
+        log(f"cpu baseline {out['cpu_baseline']['value']:.3f} frames/s")
+
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -195,7 +195,10 @@ typedef struct smip_job_stats {
     double synth_ms;        /* device time of image synthesis */
     uint64_t footprint;     /* device bytes held by the job */
     uint64_t reco_calls;    /* calc_phase invocations */
-    uint64_t cells_per_frame;
+    uint64_t cells_per_frame;   /* accumulator cells the tile kernel computes per frame */
+    uint64_t triples_per_frame; /* (u,v) tuples the reference loop adds per frame */
+    double tile_ms;             /* device time of the triple-product kernel launches */
+    uint64_t tile_launches;     /* number of those launches */
 } smip_job_stats;
 
 /* Device bytes a job of these parameters holds (admission control). */

@@ -127,7 +127,7 @@ struct smip_frames {
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
-    int kind = 0; // 0 accumulate, 1 saa, 2 reco, 3 synth
+    int kind = 0; // 0 accumulate, 1 saa, 2 reco, 3 synth, 4 triple-product kernel
 };
 
 struct smip_job {
@@ -160,7 +160,9 @@ struct smip_job {
     uint64_t reco_calls = 0;
     std::unique_ptr<smip_bispec> view;
     std::vector<EventPair> events;
-    double ms[4] = { 0, 0, 0, 0 };
+    double ms[5] = { 0, 0, 0, 0, 0 };
+    uint64_t tile_launches = 0;
+    uint64_t triples = 0;
     uint64_t footprint = 0;
     ~smip_job()
     {
@@ -959,6 +961,23 @@ JobSizes job_sizes(const smip_job_params& p)
 
 void record(smip_job* j, int kind, hipEvent_t a, hipEvent_t b) { j->events.push_back({ a, b, kind }); }
 
+// (u, v) tuples the accumulate loop of include/bispectrum.h:393-398 adds per frame
+uint64_t count_triples(const LoopBounds& L)
+{
+    uint64_t n = 0;
+    for (int32_t i = L.min1; i <= 0; ++i)
+        for (int32_t k = L.min3; k <= 0; ++k) {
+            if (i + k < L.min1 || i + k > L.max1)
+                continue;
+            for (int32_t j = L.min2; j <= L.max2; ++j) {
+                const int32_t lo = std::max(L.min4, L.min2 - j), hi = std::min(L.max4, L.max2 - j);
+                if (hi >= lo)
+                    n += (uint64_t)(hi - lo + 1);
+            }
+        }
+    return n;
+}
+
 hipEvent_t mark(smip_job* j)
 {
     hipEvent_t e;
@@ -994,7 +1013,11 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
         if (mixed) {
             launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H32.get(), j->work32.get(),
                 j->P_half.get(), s);
+            hipEvent_t t0 = mark(j);
             launch_accum_tile(j->tile, j->hp, j->H32.get(), nf, j->B64.get(), s);
+            hipEvent_t t1 = mark(j);
+            record(j, 4, t0, t1);
+            j->tile_launches++;
         } else {
             launch_forward_f64(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H64.get(), j->work64.get(),
                 j->P_half.get(), s);
@@ -1074,6 +1097,7 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         j->hp = z.hp;
         j->batch = z.batch;
         j->footprint = z.bytes;
+        j->triples = count_triples(z.L);
         SMIP_HIP(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
         hipStream_t s = j->stream;
         const uint64_t X = p->ncols, Y = p->nrows, hx = X / 2 + 1, np = X * Y;
@@ -1220,6 +1244,9 @@ extern "C" SMIP_API int smip_job_get_stats(smip_job* j, smip_job_stats* out)
         out->footprint = j->footprint;
         out->reco_calls = j->reco_calls;
         out->cells_per_frame = j->tile ? tile_plan_cells(j->tile) : 0;
+        out->triples_per_frame = j->triples;
+        out->tile_ms = j->ms[4];
+        out->tile_launches = j->tile_launches;
     });
 }
 
