@@ -606,12 +606,12 @@ int so_reconstruct_phases(const so_desc* d, const void* B, int is_f32,
     so_reco_stats st;
     memset(&st, 0, sizeof(st));
 
-    /* seeds, :42-54 (init_phase = 1+0i, its conjugate is the same value) */
+    /* seeds, :42-54: init_phase = 1+0i; std::conj(init_phase) = 1-0i at (-1,0), (0,-1) */
     const int32_t seeds[5][2] = { { 0, 0 }, { 1, 0 }, { 0, 1 }, { -1, 0 }, { 0, -1 } };
     for (int s = 0; s < 5; ++s) {
         uint64_t a = so_pm_at(&pm, seeds[s][0], seeds[s][1]);
         phases[2 * a] = 1.0;
-        phases[2 * a + 1] = 0.0;
+        phases[2 * a + 1] = s >= 3 ? -0.0 : 0.0;
         flags[a] = 1;
         consistency[a] = 1.0;
     }

@@ -5,14 +5,28 @@
 // literal restatement of NextRecoIndex (src/phasereco.cpp:17-36); only visits
 // inside the PhaseMap range are uploaded (the others do nothing).
 //
-// The recursion is sequential: each calc_phase reads the phases of points set by
-// earlier calls.  One workgroup walks the visit list; for every unflagged point w
-// its threads evaluate the candidate pairs (u, v = w - u), v in the stored v range,
-// in parallel, reduce the unit phasors in FP64 with a fixed tree, and thread 0
-// sets flag, consistency and phase exactly as calc_phase does.  Eligibility
-// (both flags set, |beta| > DBL_EPSILON) is evaluated exactly, so the flag mask
-// is bit-exact; only the summation order of the FP64 mean differs.
+// calc_phase(w) averages, over every u with v = w - u in the stored v range and
+// both flags set *at the time of the call*, the unit phasors
+// phi(u) phi(v) conj(beta / |beta|) (|beta| > DBL_EPSILON), then sets flag,
+// consistency and phase of w.  The recursion is sequential, so the schedule
+// splits the visit list into windows of K visits:
+//
+//  * k_reco_bulk (K workgroups, one per visit of the window) sums the pairs
+//    whose u and v were both flagged before the window -- exact, since flags
+//    are never cleared -- and writes a "pending" record for every pair in which
+//    u or v is a point visited earlier in the same window (it may get flagged
+//    before this visit): beta-hat, the pre-window flags and phases, and the
+//    window slots of the in-window points.  At most 2(K-1) records per visit.
+//  * k_reco_fixup (one wavefront) walks the window's visits in order; each step
+//    adds the pending pairs whose points are flagged by now, reading only LDS,
+//    and finalises the point exactly as calc_phase does.
+//
+// Eligibility is evaluated exactly, so the flag mask is bit-exact; only the
+// order of the FP64 summation of the mean differs from the reference's raster
+// order.  k_reco_sequential keeps the one-visit-at-a-time form as a cross-check.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "smip_complex.h"
@@ -21,6 +35,9 @@
 namespace smip_amd {
 
 constexpr double kEps = 2.220446049250313e-16; // constants::c_epsilon<double>
+constexpr int kWindow = 256;                   // visits per window
+constexpr int kBulkThreads = 256;
+constexpr int kFixBuf = 512; // pending records staged in LDS per chunk
 
 struct RecoArgs {
     Desc d;
@@ -29,10 +46,31 @@ struct RecoArgs {
     int64_t nvisits;
 };
 
+enum : int32_t { kPreU = 1, kPreV = 2, kValid = 4, kErr = 8 };
+
+// c = conj(beta/|beta|) times the pre-window phase of whichever of u, v was
+// flagged before the window (records never have both); the fixup multiplies in
+// the phase(s) of the in-window point(s).
+struct __align__(16) PendingRec {
+    int32_t ju, jv; // window slots of u / v (-1: not an in-window point)
+    int32_t f;      // kPreU | kPreV | kValid | kErr
+    int32_t pad;
+    double2 c;
+};
+
 struct RecoWork {
     RecoArgs args;
+    int64_t nwin = 0;
+    int cap = 0; // records per visit
     int2* visits = nullptr;
-    int32_t* status = nullptr;  // device: [0] error code, [1] calls, [2] flagged
+    int32_t* first_pos = nullptr; // window slot of the first visit of the same point
+    int64_t* map = nullptr;       // point -> (window << 32) | slot of its first visit
+    double2* S = nullptr;
+    int32_t* N = nullptr; // -1: the visit does nothing
+    int32_t* E = nullptr;
+    int32_t* cnt = nullptr;
+    PendingRec* rec = nullptr;
+    int32_t* status = nullptr; // [0] error code, [1] calls, [2] flagged
 };
 
 __device__ inline int32_t pm_at(const RecoArgs& a, int32_t x, int32_t y)
@@ -42,15 +80,460 @@ __device__ inline int32_t pm_at(const RecoArgs& a, int32_t x, int32_t y)
     return yy * a.X + xx;
 }
 
-constexpr int kRecoThreads = 1024;
+__device__ inline bool calc_phase_returns_early(const RecoArgs& a, int2 w)
+{
+    // include/phasereco.h:84-86
+    return abs(w.x) + abs(w.y) <= 1 || w.x < a.d.min_idx[0] || w.x > a.d.max_idx[0] || w.y < a.d.min_idx[1]
+        || w.y > a.d.max_idx[1];
+}
 
-__global__ __launch_bounds__(kRecoThreads) void k_reco_simple(RecoArgs a, const int2* __restrict__ visits,
+__device__ inline double2 load_beta(const void* B, int is_f32, uint64_t addr, bool cj)
+{
+    double2 beta;
+    if (is_f32) {
+        const float2 f = static_cast<const float2*>(B)[addr];
+        beta = mk<double2>(f.x, f.y);
+    } else {
+        beta = static_cast<const double2*>(B)[addr];
+    }
+    if (cj)
+        beta.y = -beta.y;
+    return beta;
+}
+
+// phi(u) phi(v) conj(beta/|beta|), normalised -- include/phasereco.h:97-105.
+// |q| as sqrt(re^2 + im^2): the operands are unit phasors, far from the range
+// where hypot's scaling matters.
+__device__ inline double2 normalised(double2 q)
+{
+    // For |q|^2 = 1 + d, 1/sqrt(1 + d) = 1 - d/2 + O(d^2): one Newton step from 1
+    // is exact to rounding when |d| < 1e-8; anything else (including q = 0,
+    // which gives NaN as the reference's 0/0 does) takes the exact path.
+    const double m2 = q.x * q.x + q.y * q.y;
+    const double inv = fabs(m2 - 1.0) < 1e-8 ? 1.5 - 0.5 * m2 : 1.0 / sqrt(m2);
+    return mk<double2>(q.x * inv, q.y * inv);
+}
+
+__device__ inline double2 unit_phasor(double2 pu, double2 pv, double2 bhat)
+{
+    return normalised(cmul(cmul(pu, pv), bhat));
+}
+
+// Sum over the 64 lanes of a wavefront with DPP row operations (quad xor 1 and
+// 2, half-row and row mirrors, then row broadcasts 15 and 31); the total ends in
+// lane 63 and is broadcast with readlane.  A few cycles per level, against an
+// LDS round trip per level for shuffle-based butterflies.
+template <int CTRL, int ROWS = 0xf>
+__device__ inline int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ inline double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffff));
+    const int hi = dpp_i<CTRL, ROWS>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double wave_sum(double v)
+{
+    v += dpp_d<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);       // row_half_mirror
+    v += dpp_d<0x140>(v);       // row_mirror
+    v += dpp_d<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+    v += dpp_d<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline int wave_sum(int v)
+{
+    v += dpp_i<0xB1>(v);
+    v += dpp_i<0x4E>(v);
+    v += dpp_i<0x141>(v);
+    v += dpp_i<0x140>(v);
+    v += dpp_i<0x142, 0xa>(v);
+    v += dpp_i<0x143, 0xc>(v);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Orders LDS accesses of one wavefront for the compiler; the hardware executes a
+// wavefront's LDS instructions in issue order, so no wait is needed.
+__device__ inline void wave_lds_order()
+{
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ inline void finalize_point(double sr, double si, int n, double2* phases, uint8_t* flags, double* cons,
+    int32_t pw, double2* out_phase)
+{
+    // mean = sum / n; consistency = |mean|; phase = mean / |mean| = sum / |sum|
+    const double r = sqrt(sr * sr + si * si);
+    const double ap = r / (double)n;
+    const double ia = 1.0 / r;
+    const double2 ph = ap > kEps ? mk<double2>(sr * ia, si * ia) : mk<double2>(0.0, 0.0);
+    flags[pw] = 1;
+    cons[pw] = ap;
+    phases[pw] = ph;
+    if (out_phase)
+        *out_phase = ph;
+}
+
+// ---------------------------------------------------------------------------
+// windowed schedule
+// ---------------------------------------------------------------------------
+__global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __restrict__ first_pos, RecoArgs a,
+    int64_t s, int64_t e, int64_t win, int64_t* __restrict__ map)
+{
+    const int64_t k = s + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= e)
+        return;
+    if (first_pos[k] == (int32_t)(k - s)) {
+        const int2 w = visits[k];
+        map[pm_at(a, w.x, w.y)] = (win << 32) | (int64_t)(k - s);
+    }
+}
+
+__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits, int64_t s,
+    int64_t win, const int64_t* __restrict__ map, const uint8_t* __restrict__ flags,
+    const double2* __restrict__ phases, const void* __restrict__ B, int is_f32, double2* __restrict__ S,
+    int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt, PendingRec* __restrict__ rec, int cap,
+    const int32_t* __restrict__ status)
+{
+    __shared__ double red_re[kBulkThreads / 64], red_im[kBulkThreads / 64];
+    __shared__ int red_n[kBulkThreads / 64], red_e[kBulkThreads / 64], red_p[kBulkThreads / 64];
+    const int j = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (status[0])
+        return; // an earlier window raised
+    const int2 w = visits[s + j];
+    const int32_t pw = pm_at(a, w.x, w.y);
+    if (flags[pw] || calc_phase_returns_early(a, w)) {
+        if (tid == 0) {
+            N[j] = -1;
+            cnt[j] = 0;
+            E[j] = 0;
+        }
+        return;
+    }
+    const Desc& d = a.d;
+    const int vmn0 = d.min_idx[2], vmn1 = d.min_idx[3];
+    const int nvx = d.max_idx[2] - vmn0 + 1, nvy = d.max_idx[3] - vmn1 + 1;
+    const int ncand = nvx * nvy;
+    double sr = 0.0, si = 0.0;
+    int n = 0, err = 0;
+    uint64_t pend = 0; // bit it: candidate tid + it * blockDim is pending
+    int it = 0;
+    for (int c = tid; c < ncand; c += blockDim.x, ++it) {
+        const int vy = vmn1 + c / nvx, vx = vmn0 + c % nvx;
+        const int ux = w.x - vx, uy = w.y - vy;
+        if (ux < a.pmn0 || ux > a.pmx0 || uy < a.pmn1 || uy > a.pmx1)
+            continue;
+        const int32_t pu = pm_at(a, ux, uy), pv = pm_at(a, vx, vy);
+        const bool fu = flags[pu] != 0, fv = flags[pv] != 0;
+        if (fu && fv) {
+            uint64_t addr;
+            bool cj;
+            if (resolve_element(d, ux, uy, vx, vy, &addr, &cj)) {
+                err = 1;
+                continue;
+            }
+            const double2 beta = load_beta(B, is_f32, addr, cj);
+            const double ab = hypot(beta.x, beta.y);
+            if (ab > kEps) {
+                const double2 q = unit_phasor(phases[pu], phases[pv], mk<double2>(beta.x / ab, -(beta.y / ab)));
+                sr += q.x;
+                si += q.y;
+                ++n;
+            }
+        } else {
+            const int64_t mu = map[pu], mv = map[pv];
+            const bool iu = (mu >> 32) == win && (int32_t)(mu & 0xffffffff) < j;
+            const bool iv = (mv >> 32) == win && (int32_t)(mv & 0xffffffff) < j;
+            if ((fu || iu) && (fv || iv))
+                pend |= 1ull << it;
+        }
+    }
+    // block reductions in a fixed order
+    int np = __popcll(pend);
+    int incl = np; // inclusive scan of pending counts over the block
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        si += __shfl_xor(si, o);
+        n += __shfl_xor(n, o);
+        err |= __shfl_xor(err, o);
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if ((tid & 63) >= o)
+            incl += t;
+    }
+    if ((tid & 63) == 63)
+        red_p[tid >> 6] = incl;
+    if ((tid & 63) == 0) {
+        red_re[tid >> 6] = sr;
+        red_im[tid >> 6] = si;
+        red_n[tid >> 6] = n;
+        red_e[tid >> 6] = err;
+    }
+    __syncthreads();
+    int base = incl - np;
+    for (int wv = 0; wv < (tid >> 6); ++wv)
+        base += red_p[wv];
+    if (tid == 0) {
+        double tr = 0.0, ti = 0.0;
+        int tn = 0, te = 0, tp = 0;
+        for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) {
+            tr += red_re[wv];
+            ti += red_im[wv];
+            tn += red_n[wv];
+            te |= red_e[wv];
+            tp += red_p[wv];
+        }
+        S[j] = mk<double2>(tr, ti);
+        N[j] = tn;
+        E[j] = te;
+        cnt[j] = tp;
+    }
+    // pending records, in (thread, candidate) order
+    PendingRec* out = rec + (int64_t)j * cap + base;
+    it = 0;
+    for (int c = tid; c < ncand; c += blockDim.x, ++it) {
+        if (!(pend & (1ull << it)))
+            continue;
+        const int vy = vmn1 + c / nvx, vx = vmn0 + c % nvx;
+        const int ux = w.x - vx, uy = w.y - vy;
+        const int32_t pu = pm_at(a, ux, uy), pv = pm_at(a, vx, vy);
+        const bool fu = flags[pu] != 0, fv = flags[pv] != 0;
+        const int64_t mu = map[pu], mv = map[pv];
+        PendingRec r;
+        r.ju = ((mu >> 32) == win && (int32_t)(mu & 0xffffffff) < j) ? (int32_t)(mu & 0xffffffff) : -1;
+        r.jv = ((mv >> 32) == win && (int32_t)(mv & 0xffffffff) < j) ? (int32_t)(mv & 0xffffffff) : -1;
+        r.f = (fu ? kPreU : 0) | (fv ? kPreV : 0);
+        r.pad = 0;
+        r.c = mk<double2>(0, 0);
+        uint64_t addr;
+        bool cj;
+        if (resolve_element(d, ux, uy, vx, vy, &addr, &cj)) {
+            r.f |= kErr;
+        } else {
+            const double2 beta = load_beta(B, is_f32, addr, cj);
+            const double ab = hypot(beta.x, beta.y);
+            if (ab > kEps) {
+                r.f |= kValid;
+                const double2 bh = mk<double2>(beta.x / ab, -(beta.y / ab));
+                r.c = fu ? cmul(phases[pu], bh) : (fv ? cmul(phases[pv], bh) : bh);
+            }
+        }
+        *out++ = r;
+    }
+}
+
+struct StepInfo {
+    int32_t n0;  // pre-window eligible count, -1: the visit does nothing
+    int32_t fp;  // window slot of the first visit of this point
+    int32_t cnt; // pending records
+    int32_t off; // offset of the records within the chunk buffer
+    int32_t pw;  // PhaseMap position of the point
+    int32_t err; // an eligible pre-window lookup was out of range
+    int32_t rev; // not the first visit of the point in this window
+    int32_t pad;
+    double2 S;   // pre-window sum
+};
+
+constexpr int kChunk = 8;        // visits per staged chunk
+constexpr int kChunkRecs = 512;  // records per chunk buffer
+constexpr int kParts = sizeof(PendingRec) / sizeof(int4);
+
+// Stage chunk c's pending records into the LDS buffer dst with direct
+// global->LDS loads (global_load_lds_dwordx4: lane i's 16 bytes land at
+// dst + 16 i).  Piece p = u*64 + lane belongs to record t = p / kParts of the
+// chunk, owned by the last visit whose chunk-local offset is <= t.  The loads
+// complete asynchronously; the caller waits (vmcnt) before reading dst.
+__device__ __forceinline__ void fix_stage_chunk(int4* dst, const int4* __restrict__ src4, const StepInfo* info, int c,
+    int nsteps, int pieces, int cap, int lane)
+{
+    const int lq0 = c * kChunk, lq1 = min(nsteps, lq0 + kChunk);
+    int offs[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+        offs[k] = lq0 + k < lq1 ? info[lq0 + k].off : (1 << 30);
+    for (int u0 = 0; u0 < pieces; u0 += 64) {
+        const int pc = u0 + lane;
+        const int t = pc / kParts, part = pc - t * kParts;
+        int k = 0, o = 0;
+#pragma unroll
+        for (int kk = 0; kk < kChunk; ++kk)
+            if (offs[kk] <= t) {
+                k = kk;
+                o = offs[kk];
+            }
+        if (pc < pieces)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void*)(src4
+                    + ((int64_t)(lq0 + k) * cap + (t - o)) * kParts + part),
+                (__attribute__((address_space(3))) void*)(dst + u0), 16, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
+    const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
+    const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
+    const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
+    double* __restrict__ cons, int32_t* __restrict__ status)
+{
+    __shared__ PendingRec buf[kChunkRecs];
+    __shared__ StepInfo info[kWindow];
+    __shared__ int32_t ctotal[kWindow / kChunk];
+    __shared__ double2 ppos[kWindow];
+    __shared__ uint8_t fpos[kWindow];
+    const int lane = threadIdx.x;
+    if (status[0])
+        return;
+    const int nsteps = (int)(e - s);
+    const int nchunks = (nsteps + kChunk - 1) / kChunk;
+    // per-visit scalars of the whole window (independent loads)
+    for (int t = lane; t < nsteps; t += 64) {
+        StepInfo si;
+        si.n0 = N[t];
+        si.fp = first_pos[s + t];
+        si.cnt = cnt[t];
+        si.err = E[t];
+        si.rev = si.fp != t;
+        const int2 w = visits[s + t];
+        si.pw = pm_at(a, w.x, w.y);
+        si.S = S[t];
+        si.off = 0;
+        si.pad = 0;
+        info[t] = si;
+        fpos[t] = 0;
+    }
+    for (int t = nsteps + lane; t < kWindow; t += 64)
+        fpos[t] = 0;
+    __syncthreads();
+    for (int c = lane; c < nchunks; c += 64) { // chunk-local record offsets
+        int o = 0;
+        for (int q = c * kChunk; q < min(nsteps, (c + 1) * kChunk); ++q) {
+            info[q].off = o;
+            o += info[q].cnt;
+        }
+        ctotal[c] = o;
+    }
+    __syncthreads();
+
+    const int4* src4 = reinterpret_cast<const int4*>(rec);
+    int calls = 0, flagged = 0;
+    bool aborted = false;
+    for (int c = 0; c < nchunks && !aborted; ++c) {
+        const bool staged = ctotal[c] <= kChunkRecs;
+        if (staged) { // one burst of direct global->LDS loads, one wait
+            fix_stage_chunk(reinterpret_cast<int4*>(buf), src4, info, c, nsteps, ctotal[c] * kParts, cap, lane);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();
+        const int q0 = c * kChunk, q1 = min(nsteps, q0 + kChunk);
+        PendingRec nr {};
+        if (staged) {
+            const StepInfo i0 = info[q0];
+            if (lane < i0.cnt)
+                nr = buf[i0.off + lane];
+        }
+        for (int j = q0; j < q1; ++j) {
+            const StepInfo si = info[j];
+            const PendingRec r0 = nr; // this visit's first 64 records, prefetched from LDS
+            if (staged && j + 1 < q1) {
+                const StepInfo sn = info[j + 1];
+                nr = PendingRec {};
+                if (lane < sn.cnt)
+                    nr = buf[sn.off + lane];
+            }
+            if (si.n0 < 0)
+                continue;
+            if (si.rev && fpos[si.fp])
+                continue; // flagged by an earlier visit of this window
+            ++calls;
+            double sr = 0.0, sim = 0.0;
+            int ne = 0; // count + (error << 16)
+            auto take = [&](const PendingRec& r) {
+                const int ju = max(r.ju, 0), jv = max(r.jv, 0);
+                const uint8_t fu = fpos[ju], fv = fpos[jv];
+                const double2 pu = ppos[ju], pv = ppos[jv];
+                const bool uok = (r.f & kPreU) || (r.ju >= 0 && fu);
+                const bool vok = (r.f & kPreV) || (r.jv >= 0 && fv);
+                if (!(uok && vok))
+                    return;
+                if (r.f & kErr) {
+                    ne += 1 << 16;
+                    return;
+                }
+                if (r.f & kValid) {
+                    double2 qv = r.c;
+                    if (!(r.f & kPreU))
+                        qv = cmul(pu, qv);
+                    if (!(r.f & kPreV))
+                        qv = cmul(qv, pv);
+                    qv = normalised(qv);
+                    sr += qv.x;
+                    sim += qv.y;
+                    ++ne;
+                }
+            };
+            if (staged) {
+                if (lane < si.cnt)
+                    take(r0);
+                for (int t = 64 + lane; t < si.cnt; t += 64) // rare: more than 64 records
+                    take(buf[si.off + t]);
+            } else {
+                for (int t = lane; t < si.cnt; t += 64)
+                    take(rec[(int64_t)j * cap + t]);
+            }
+            ne = wave_sum(ne);
+            if ((ne >> 16) || si.err) {
+                if (lane == 0)
+                    status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
+                aborted = true;
+                break;
+            }
+            const int tn = si.n0 + (ne & 0xffff);
+            if (tn > 0) {
+                sr = wave_sum(sr);
+                sim = wave_sum(sim);
+                if (lane == 0) {
+                    double2 ph;
+                    finalize_point(si.S.x + sr, si.S.y + sim, tn, phases, flags, cons, si.pw, &ph);
+                    ppos[si.fp] = ph;
+                    fpos[si.fp] = 1;
+                }
+                ++flagged;
+                wave_lds_order();
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicAdd(&status[1], calls);
+        atomicAdd(&status[2], flagged);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one visit at a time (cross-check form)
+// ---------------------------------------------------------------------------
+constexpr int kSeqThreads = 1024;
+
+__global__ __launch_bounds__(kSeqThreads) void k_reco_sequential(RecoArgs a, const int2* __restrict__ visits,
     const void* __restrict__ B, int is_f32, double2* __restrict__ phases, uint8_t* __restrict__ flags,
     double* __restrict__ cons, int32_t* __restrict__ status)
 {
-    extern __shared__ uint32_t bits[]; // PhaseMap flags, one bit per point
-    __shared__ double red_re[kRecoThreads / 64], red_im[kRecoThreads / 64];
-    __shared__ int red_n[kRecoThreads / 64];
+    extern __shared__ uint32_t bits[];
+    __shared__ double red_re[kSeqThreads / 64], red_im[kSeqThreads / 64];
+    __shared__ int red_n[kSeqThreads / 64];
     __shared__ int err;
     const int tid = threadIdx.x;
     const int npts = a.X * a.Y;
@@ -67,21 +550,17 @@ __global__ __launch_bounds__(kRecoThreads) void k_reco_simple(RecoArgs a, const 
     if (tid == 0)
         err = 0;
     __syncthreads();
-
     const Desc& d = a.d;
-    const int vmn0 = d.min_idx[2], vmx0 = d.max_idx[2], vmn1 = d.min_idx[3], vmx1 = d.max_idx[3];
-    const int nvx = vmx0 - vmn0 + 1, nvy = vmx1 - vmn1 + 1;
+    const int vmn0 = d.min_idx[2], vmn1 = d.min_idx[3];
+    const int nvx = d.max_idx[2] - vmn0 + 1, nvy = d.max_idx[3] - vmn1 + 1;
     const int ncand = nvx * nvy;
     int calls = 0, flagged = 0;
-
     for (int64_t vi = 0; vi < a.nvisits; ++vi) {
         const int2 w = visits[vi];
         const int wpos = pm_at(a, w.x, w.y);
         if (bits[wpos >> 5] & (1u << (wpos & 31)))
             continue;
-        // calc_phase's early return, include/phasereco.h:84-86
-        if (abs(w.x) + abs(w.y) <= 1 || w.x < d.min_idx[0] || w.x > d.max_idx[0] || w.y < d.min_idx[1]
-            || w.y > d.max_idx[1])
+        if (calc_phase_returns_early(a, w))
             continue;
         ++calls;
         double sr = 0.0, si = 0.0;
@@ -97,31 +576,18 @@ __global__ __launch_bounds__(kRecoThreads) void k_reco_simple(RecoArgs a, const 
             uint64_t addr;
             bool cj;
             if (resolve_element(d, ux, uy, vx, vy, &addr, &cj)) {
-                err = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
+                err = SMIP_ERR_OUT_OF_RANGE;
                 continue;
             }
-            double2 beta;
-            if (is_f32) {
-                const float2 f = static_cast<const float2*>(B)[addr];
-                beta = mk<double2>(f.x, f.y);
-            } else {
-                beta = static_cast<const double2*>(B)[addr];
-            }
-            if (cj)
-                beta.y = -beta.y;
-            // ph = phases(u) * phases(v); if |beta| > eps: ph *= conj(beta/|beta|); push ph/|ph|
-            const double2 ph = cmul(phases[pu], phases[pv]);
+            const double2 beta = load_beta(B, is_f32, addr, cj);
             const double ab = hypot(beta.x, beta.y);
             if (ab > kEps) {
-                const double2 t = mk<double2>(beta.x / ab, -(beta.y / ab));
-                const double2 q = cmul(ph, t);
-                const double aq = hypot(q.x, q.y);
-                sr += q.x / aq;
-                si += q.y / aq;
+                const double2 q = unit_phasor(phases[pu], phases[pv], mk<double2>(beta.x / ab, -(beta.y / ab)));
+                sr += q.x;
+                si += q.y;
                 ++cnt;
             }
         }
-        // fixed-order reduction: wave butterflies, then the waves in order
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             sr += __shfl_xor(sr, o);
@@ -144,12 +610,7 @@ __global__ __launch_bounds__(kRecoThreads) void k_reco_simple(RecoArgs a, const 
             }
             if (tn > 0 && err == 0) {
                 bits[wpos >> 5] |= 1u << (wpos & 31);
-                flags[wpos] = 1;
-                tr /= (double)tn;
-                ti /= (double)tn;
-                const double ap = hypot(tr, ti);
-                cons[wpos] = ap;
-                phases[wpos] = ap > kEps ? mk<double2>(tr / ap, ti / ap) : mk<double2>(0.0, 0.0);
+                finalize_point(tr, ti, tn, phases, flags, cons, wpos, nullptr);
                 ++flagged;
             }
         }
@@ -164,6 +625,29 @@ __global__ __launch_bounds__(kRecoThreads) void k_reco_simple(RecoArgs a, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+static T* dalloc(size_t n)
+{
+    T* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(SMIP_ERR_NO_MEMORY, "phase-recovery work buffers do not fit in device memory");
+    }
+    return p;
+}
+
+uint64_t reco_work_bytes(const Desc& d, int32_t xsize, int32_t ysize, int64_t nvisits)
+{
+    const int ncand = (d.max_idx[2] - d.min_idx[2] + 1) * (d.max_idx[3] - d.min_idx[3] + 1);
+    const int cap = std::min(ncand, 2 * kWindow);
+    return (uint64_t)nvisits * (sizeof(int2) + sizeof(int32_t)) + (uint64_t)xsize * ysize * sizeof(int64_t)
+        + (uint64_t)kWindow * (sizeof(double2) + 4 * sizeof(int32_t))
+        + (uint64_t)kWindow * cap * sizeof(PendingRec);
+}
+
 RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits)
 {
     auto* w = new RecoWork;
@@ -176,11 +660,42 @@ RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const in
     a.pmx0 = -xsize / 2 + xsize - 1;
     a.pmx1 = -ysize / 2 + ysize - 1;
     a.nvisits = nvisits;
+    const int ncand = (d.max_idx[2] - d.min_idx[2] + 1) * (d.max_idx[3] - d.min_idx[3] + 1);
+    if (ncand > 64 * kBulkThreads) {
+        delete w;
+        fail(SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth too large for phase recovery (b/2 <= 63)");
+    }
+    w->cap = std::min(ncand, 2 * kWindow);
+    w->nwin = (nvisits + kWindow - 1) / kWindow;
+    // window slot of the first visit of each visit's point within its window
+    std::vector<int32_t> first(std::max<int64_t>(nvisits, 1));
+    {
+        std::vector<int32_t> seen((size_t)xsize * ysize, -1);
+        for (int64_t s = 0; s < nvisits; s += kWindow) {
+            const int64_t e = std::min<int64_t>(s + kWindow, nvisits);
+            for (int64_t k = s; k < e; ++k) {
+                int32_t x = visits_host[2 * k], y = visits_host[2 * k + 1];
+                const size_t p = (size_t)((y < 0 ? y + ysize : y) * xsize + (x < 0 ? x + xsize : x));
+                if (seen[p] < s) // not yet visited in this window
+                    seen[p] = (int32_t)k;
+                first[k] = seen[p] - (int32_t)s;
+            }
+        }
+    }
     try {
-        SMIP_HIP(hipMalloc(&w->visits, sizeof(int2) * std::max<int64_t>(nvisits, 1)));
-        SMIP_HIP(hipMalloc(&w->status, sizeof(int32_t) * 4));
-        if (nvisits)
+        w->visits = dalloc<int2>(nvisits);
+        w->first_pos = dalloc<int32_t>(nvisits);
+        w->map = dalloc<int64_t>((size_t)xsize * ysize);
+        w->S = dalloc<double2>(kWindow);
+        w->N = dalloc<int32_t>(kWindow);
+        w->E = dalloc<int32_t>(kWindow);
+        w->cnt = dalloc<int32_t>(kWindow);
+        w->rec = dalloc<PendingRec>((size_t)kWindow * w->cap);
+        w->status = dalloc<int32_t>(4);
+        if (nvisits) {
             SMIP_HIP(hipMemcpy(w->visits, visits_host, sizeof(int2) * nvisits, hipMemcpyHostToDevice));
+            SMIP_HIP(hipMemcpy(w->first_pos, first.data(), sizeof(int32_t) * nvisits, hipMemcpyHostToDevice));
+        }
     } catch (...) {
         reco_work_destroy(w);
         throw;
@@ -192,34 +707,34 @@ void reco_work_destroy(RecoWork* w)
 {
     if (!w)
         return;
-    (void)hipFree(w->visits);
-    (void)hipFree(w->status);
+    for (void* p : { (void*)w->visits, (void*)w->first_pos, (void*)w->map, (void*)w->S, (void*)w->N, (void*)w->E,
+             (void*)w->cnt, (void*)w->rec, (void*)w->status })
+        if (p)
+            (void)hipFree(p);
     delete w;
 }
 
-// Seeds of reconstruct_phases (include/phasereco.h:42-54).
-__global__ void k_reco_seed(RecoArgs a, double2* phases, uint8_t* flags, double* cons)
+// Seeds of reconstruct_phases (include/phasereco.h:39-54) on a zeroed map.
+__global__ void k_reco_init(RecoArgs a, double2* phases, uint8_t* flags, double* cons, int64_t* map)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int npts = a.X * a.Y;
-    if (i < npts) {
-        phases[i] = mk<double2>(0, 0);
-        flags[i] = 0;
-        cons[i] = 0;
-    }
+    if (i >= a.X * a.Y)
+        return;
+    const int32_t col = i % a.X, row = i / a.X;
+    const int32_t x = col > a.pmx0 ? col - a.X : col, y = row > a.pmx1 ? row - a.Y : row;
+    const bool seed = (x == 0 && y == 0) || (abs(x) + abs(y) == 1 && (x == 0 || y == 0));
+    // init_phase = (1, 0) at (0,0), (1,0), (0,1); std::conj(init_phase) = (1, -0) at (-1,0), (0,-1)
+    phases[i] = seed ? mk<double2>(1.0, (x < 0 || y < 0) ? -0.0 : 0.0) : mk<double2>(0.0, 0.0);
+    flags[i] = seed ? 1 : 0;
+    cons[i] = seed ? 1.0 : 0.0;
+    if (map)
+        map[i] = -1;
 }
 
-__global__ void k_reco_seed2(RecoArgs a, double2* phases, uint8_t* flags, double* cons)
+__global__ void k_zero_status(int32_t* st)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    const int sx[5] = { 0, 1, 0, -1, 0 }, sy[5] = { 0, 0, 1, 0, -1 };
-    for (int s = 0; s < 5; ++s) {
-        const int p = pm_at(a, sx[s], sy[s]);
-        phases[p] = mk<double2>(1.0, 0.0);
-        flags[p] = 1;
-        cons[p] = 1.0;
-    }
+    if (threadIdx.x < 4)
+        st[threadIdx.x] = 0;
 }
 
 int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,
@@ -227,15 +742,31 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
 {
     const RecoArgs& a = w->args;
     const int npts = a.X * a.Y;
-    hipLaunchKernelGGL(k_reco_seed, dim3((npts + 255) / 256), dim3(256), 0, s, a, phases, flags, consistency);
-    hipLaunchKernelGGL(k_reco_seed2, dim3(1), dim3(1), 0, s, a, phases, flags, consistency);
+    hipLaunchKernelGGL(k_reco_init, dim3((npts + 255) / 256), dim3(256), 0, s, a, phases, flags, consistency,
+        w->map);
+    hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w->status);
     SMIP_HIP(hipGetLastError());
-    const size_t lds = sizeof(uint32_t) * ((npts + 31) / 32);
-    if (lds > 96 * 1024)
-        fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for the LDS flag bitmap");
-    hipLaunchKernelGGL(k_reco_simple, dim3(1), dim3(kRecoThreads), lds, s, a, w->visits, B, is_f32, phases, flags,
-        consistency, w->status);
-    SMIP_HIP(hipGetLastError());
+    const char* seq = std::getenv("SMIP_RECO_SEQUENTIAL");
+    if (seq && seq[0] == '1') {
+        const size_t lds = sizeof(uint32_t) * ((npts + 31) / 32);
+        if (lds > 96 * 1024)
+            fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for the sequential form");
+        hipLaunchKernelGGL(k_reco_sequential, dim3(1), dim3(kSeqThreads), lds, s, a, w->visits, B, is_f32, phases,
+            flags, consistency, w->status);
+        SMIP_HIP(hipGetLastError());
+    } else {
+        for (int64_t win = 0; win < w->nwin; ++win) {
+            const int64_t s0 = win * kWindow, e0 = std::min<int64_t>(s0 + kWindow, a.nvisits);
+            const int nk = (int)(e0 - s0);
+            hipLaunchKernelGGL(k_reco_map, dim3((nk + 255) / 256), dim3(256), 0, s, w->visits, w->first_pos, a, s0,
+                e0, win, w->map);
+            hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, s0, win, w->map, flags,
+                phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
+            hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(64), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
+                w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status);
+        }
+        SMIP_HIP(hipGetLastError());
+    }
     int32_t st[4] = { 0, 0, 0, 0 };
     SMIP_HIP(hipMemcpyAsync(st, w->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
     SMIP_HIP(hipStreamSynchronize(s));

@@ -9,7 +9,7 @@
 #include <stdexcept>
 #include <string>
 
-#include "../../include/smip_mi355x.h"
+#include "smip_mi355x.h"
 
 namespace smip_amd {
 

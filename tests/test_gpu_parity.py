@@ -179,3 +179,53 @@ def test_tile_shapes(ctx, vy_tile):
     got = S.run_pipeline(ctx, frames, b, 20, precision=S.PRECISION_MIXED, saa=False, vy_tile=vy_tile)
     assert _rel(got.bispectrum, B) < 2e-5
     assert np.array_equal(got.bispectrum == 0, B == 0)
+
+
+def test_reco_windowed_matches_sequential(ctx, monkeypatch):
+    """The windowed schedule against the one-visit-at-a-time kernel and the oracle:
+    identical flags, phases equal up to the FP64 summation order."""
+    N, b, p = 128, 32, 64
+    frames = _frames(3, N, seed=17).astype(np.float64)
+    B, _ = O.power_and_bispectrum(frames, b)
+    gpu = S.Bispectrum(ctx, (N, N, b, b))
+    gpu.upload(B)
+    ph_w, fl_w, co_w = gpu.reconstruct_phases(N, N, p)
+    monkeypatch.setenv("SMIP_RECO_SEQUENTIAL", "1")
+    ph_s, fl_s, co_s = gpu.reconstruct_phases(N, N, p)
+    monkeypatch.delenv("SMIP_RECO_SEQUENTIAL")
+    ref = O.reconstruct_phases(O.desc(N, N, b), B, N, N, p)
+    assert np.array_equal(fl_w, fl_s) and np.array_equal(fl_w, ref.flags)
+    assert np.max(np.abs(ph_w - ph_s)) < 1e-10
+    assert np.max(np.abs(ph_w - ref.phases)) < 1e-10
+    assert np.max(np.abs(co_w - ref.consistency)) < 1e-10
+
+
+@pytest.mark.parametrize("precision", [S.PRECISION_MIXED, S.PRECISION_FP64])
+def test_binary128_full_config(ctx, precision):
+    """40 frames 128x128 u8 binary star, b=32, p=64: the reference's CPU-runnable case."""
+    W = S.WORKLOADS["binary128"]
+    frames = S.synth_frames(S.synth_params("binary128"), 0, W["n"])
+    want = O.pipeline(frames.astype(np.float64), W["b"], W["p"])
+    got = S.run_pipeline(ctx, frames, W["b"], W["p"], precision=precision)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    tol_b = 2e-5 if precision == S.PRECISION_MIXED else 1e-11
+    assert _rel(got.bispectrum, want.bispectrum) < tol_b
+    assert _rel_l2(got.image, want.image) < (1e-5 if precision == S.PRECISION_MIXED else 1e-9)
+    assert got.stats["reco_calls"] == want.stats["calls"]
+
+
+def test_triple256_sample(ctx):
+    """8 frames of the 256x256 u16 triple-star workload at b=48, p=128 (the metric's
+    shape) through the mixed engine against the FP64 oracle."""
+    W = S.WORKLOADS["triple256"]
+    frames = S.synth_frames(S.synth_params("triple256"), 0, 8)
+    want = O.pipeline(frames.astype(np.float64), W["b"], W["p"])
+    got = S.run_pipeline(ctx, frames, W["b"], W["p"], precision=S.PRECISION_MIXED)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.flags, want.flags)
+    assert _rel(got.bispectrum, want.bispectrum) < 2e-5
+    err = _rel_l2(got.image, want.image)
+    assert err < 1e-5, err
