@@ -194,7 +194,7 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {n} frames {N}x{N} {np.dtype(W['dtype']).name}, "
                                f"triple star, b={b}, p={pr}, mixed precision (FP32 batch, FP64 fold)",
                    "frames": n, "ncols": N, "nrows": N, "bdepth": b, "reco_radius": pr,
-                   "frames_per_batch": args.batch or 256, "triples_per_frame": triples},
+                   "frames_per_batch": args.batch or 512, "triples_per_frame": triples},
         "plain_fp64_frames_per_s": round(fps64, 1),
         "speedup_vs_plain_fp64": round(fps / fps64, 2),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_FP32_PEAK_TFLOPS,

@@ -183,7 +183,7 @@ typedef struct smip_job_params {
     int32_t frame_dtype;       /* SMIP_DTYPE_U8 / U16 / F32 */
     int32_t precision;         /* SMIP_PRECISION_FP64 or SMIP_PRECISION_MIXED */
     int32_t saa;               /* 1: cross-correlation shift-and-add against frame 0 */
-    int32_t frames_per_batch;  /* FP32 frames per FP64 fold (MIXED); 0 = default */
+    int32_t frames_per_batch;  /* FP32 frames per FP64 fold (MIXED); 0 = default (512) */
     int32_t vy_tile;           /* vy values per register tile (MIXED); 0 = default */
 } smip_job_params;
 

@@ -5,6 +5,7 @@ Produces, next to the sources:
   speckle-stacker_amd/lib/libsmip_synth.so    host-only synthetic speckle frames
   speckle-stacker_amd/lib/smip_cli            smip-cli compatible driver
   speckle-stacker_amd/lib/smip_dropin_test    C++ drop-in surface test runner
+  speckle-stacker_amd/lib/smip_copy_calib     copy kernel calibrating rocprofv3 byte counters
 
 Run ``python speckle-stacker_amd/build.py`` or call ``build()``.
 """
@@ -90,6 +91,12 @@ def build_cli() -> list[str]:
     return built
 
 
+def build_calib() -> str:
+    out = os.path.join(LIB, "smip_copy_calib")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-x", "hip", os.path.join(CSRC, "copy_calib.cpp"), "-o", out])
+    return out
+
+
 def build() -> None:
     os.makedirs(LIB, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=2) as ex:
@@ -98,6 +105,7 @@ def build() -> None:
         f_eng.result()
         f_syn.result()
     build_cli()
+    build_calib()
 
 
 if __name__ == "__main__":

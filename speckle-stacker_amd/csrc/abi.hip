@@ -943,7 +943,7 @@ JobSizes job_sizes(const smip_job_params& p)
     const int pad = mixed ? tile_plan_pad_needed(z.d, z.vy_tile) : 0;
     const int tail = mixed ? tile_plan_tail_needed(z.d, (int)p.nrows, z.vy_tile) : 0;
     z.hp = make_half_plane((int32_t)p.ncols, (int32_t)p.nrows, pad, tail);
-    z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (mixed ? 256 : 16);
+    z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (mixed ? 512 : 16);
     const uint64_t X = p.ncols, Y = p.nrows, hx = X / 2 + 1, np = X * Y;
     uint64_t b = z.d.base_size * sizeof(double2) + hx * Y * sizeof(double);
     if (mixed)
