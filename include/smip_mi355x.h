@@ -134,6 +134,11 @@ SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* src);
  * Any output pointer may be NULL. */
 SMIP_API int smip_reconstruct_phases(const smip_bispec* b, uint64_t xsize, uint64_t ysize, double reco_radius,
     double* phases, uint8_t* flags, double* consistency);
+/* calc_phase<complex<double>, U>(bispec, phases, phasemap, w),
+ * include/phasereco.h:23-27,75-123: one recursion step on caller-held state.
+ * phases / flags / consistency are read and updated in place (xsize*ysize). */
+SMIP_API int smip_calc_phase(const smip_bispec* b, uint64_t xsize, uint64_t ysize, double* phases, uint8_t* flags,
+    double* consistency, int32_t wx, int32_t wy);
 /* NextRecoIndex, src/phasereco.cpp:17-36 (host function). */
 SMIP_API void smip_next_reco_index(double* r, double* phi, int32_t* i, int32_t* j);
 

@@ -744,6 +744,32 @@ extern "C" SMIP_API int smip_reconstruct_phases(const smip_bispec* b, uint64_t x
     });
 }
 
+extern "C" SMIP_API int smip_calc_phase(const smip_bispec* b, uint64_t xsize, uint64_t ysize, double* phases,
+    uint8_t* flags, double* consistency, int32_t wx, int32_t wy)
+{
+    return guarded([&] {
+        require(b && phases && flags && consistency, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        check_reco_geometry(b->d, xsize, ysize);
+        ensure_device(b->ctx);
+        hipStream_t s = b->ctx->stream;
+        const uint64_t n = xsize * ysize;
+        auto ph = dmake<double2>(n);
+        auto fl = dmake<uint8_t>(n);
+        auto co = dmake<double>(n);
+        SMIP_HIP(hipMemcpyAsync(ph.get(), phases, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+        SMIP_HIP(hipMemcpyAsync(fl.get(), flags, n, hipMemcpyHostToDevice, s));
+        SMIP_HIP(hipMemcpyAsync(co.get(), consistency, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (b->d.base_size == 0)
+            fail(SMIP_ERR_OUT_OF_RANGE, "Bispectrum: element address out of bounds.");
+        run_calc_phase(b->d, (int32_t)xsize, (int32_t)ysize, b->data, b->dtype == SMIP_DTYPE_C64, ph.get(), fl.get(),
+            co.get(), wx, wy, s);
+        SMIP_HIP(hipMemcpyAsync(phases, ph.get(), sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipMemcpyAsync(flags, fl.get(), n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipMemcpyAsync(consistency, co.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
+    });
+}
+
 // ---------------------------------------------------------------------------
 // CrossCorrelation<double>
 // ---------------------------------------------------------------------------

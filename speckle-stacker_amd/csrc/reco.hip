@@ -529,7 +529,7 @@ constexpr int kSeqThreads = 1024;
 
 __global__ __launch_bounds__(kSeqThreads) void k_reco_sequential(RecoArgs a, const int2* __restrict__ visits,
     const void* __restrict__ B, int is_f32, double2* __restrict__ phases, uint8_t* __restrict__ flags,
-    double* __restrict__ cons, int32_t* __restrict__ status)
+    double* __restrict__ cons, int32_t* __restrict__ status, int force)
 {
     extern __shared__ uint32_t bits[];
     __shared__ double red_re[kSeqThreads / 64], red_im[kSeqThreads / 64];
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kSeqThreads) void k_reco_sequential(RecoArgs a, con
     for (int64_t vi = 0; vi < a.nvisits; ++vi) {
         const int2 w = visits[vi];
         const int wpos = pm_at(a, w.x, w.y);
-        if (bits[wpos >> 5] & (1u << (wpos & 31)))
+        if (!force && (bits[wpos >> 5] & (1u << (wpos & 31))))
             continue;
         if (calc_phase_returns_early(a, w))
             continue;
@@ -752,7 +752,7 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
         if (lds > 96 * 1024)
             fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for the sequential form");
         hipLaunchKernelGGL(k_reco_sequential, dim3(1), dim3(kSeqThreads), lds, s, a, w->visits, B, is_f32, phases,
-            flags, consistency, w->status);
+            flags, consistency, w->status, 0);
         SMIP_HIP(hipGetLastError());
     } else {
         for (int64_t win = 0; win < w->nwin; ++win) {
@@ -773,6 +773,34 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
     if (st[0])
         fail(st[0], "reconstruct_phases: bispectrum element address out of bounds");
     return st[1];
+}
+
+// One calc_phase(w) on caller-provided state (include/phasereco.h:75-123): the
+// point is evaluated whatever its flag, as calc_phase itself does not look at it.
+void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
+    uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s)
+{
+    const int32_t v[2] = { wx, wy };
+    RecoWork* w = reco_work_create(d, xsize, ysize, v, 1);
+    try {
+        const int npts = xsize * ysize;
+        hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w->status);
+        const size_t lds = sizeof(uint32_t) * ((npts + 31) / 32);
+        if (lds > 96 * 1024)
+            fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for calc_phase");
+        hipLaunchKernelGGL(k_reco_sequential, dim3(1), dim3(kSeqThreads), lds, s, w->args, w->visits, B, is_f32,
+            phases, flags, consistency, w->status, 1);
+        SMIP_HIP(hipGetLastError());
+        int32_t st[4] = { 0, 0, 0, 0 };
+        SMIP_HIP(hipMemcpyAsync(st, w->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
+        if (st[0])
+            fail(st[0], "calc_phase: bispectrum element address out of bounds");
+    } catch (...) {
+        reco_work_destroy(w);
+        throw;
+    }
+    reco_work_destroy(w);
 }
 
 } // namespace smip_amd

@@ -212,6 +212,8 @@ struct RecoWork;
 RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host,
     int64_t nvisits);
 void reco_work_destroy(RecoWork*);
+void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
+    uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s);
 int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,
     hipStream_t s);
 

@@ -88,6 +88,7 @@ _SIGS = {
     "smip_bispec_array_op": (C.c_int, [_P, _P, C.c_int32]),
     "smip_bispec_assign": (C.c_int, [_P, _P]),
     "smip_reconstruct_phases": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_double, _P, _P, _P]),
+    "smip_calc_phase": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int32, C.c_int32]),
     "smip_next_reco_index": (None, [_P, _P, _P, _P]),
     "smip_xcorr_create": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64, _P]),
     "smip_xcorr_destroy": (C.c_int, [_P]),
@@ -268,6 +269,14 @@ class Bispectrum:
 
     def assign(self, other: "Bispectrum") -> None:
         _check(lib().smip_bispec_assign(self.h, other.h))
+
+    def calc_phase(self, phases: np.ndarray, flags: np.ndarray, consistency: np.ndarray, w) -> None:
+        """One calc_phase step on (phases, flags, consistency), updated in place."""
+        for a, dt in ((phases, np.complex128), (flags, np.uint8), (consistency, np.float64)):
+            if a.dtype != dt or not a.flags.c_contiguous:
+                raise SmipError(ERR_INVALID_ARGUMENT, "calc_phase state arrays must be contiguous")
+        ys, xs = phases.shape
+        _check(lib().smip_calc_phase(self.h, xs, ys, _p(phases), _p(flags), _p(consistency), int(w[0]), int(w[1])))
 
     def reconstruct_phases(self, xsize: int, ysize: int, reco_radius: float):
         ph = np.zeros((ysize, xsize), dtype=np.complex128)
