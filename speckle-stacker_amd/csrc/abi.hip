@@ -19,26 +19,36 @@ using namespace smip_amd;
 
 namespace {
 
-thread_local std::string g_last_error;
+// Trivially destructible on purpose: callers may reach the ABI from static
+// destructors, which run after the calling thread's thread_local objects have
+// been destroyed; a std::string here would then be written after its heap
+// buffer was freed.
+thread_local char g_last_error[512];
+
+void set_last_error(const char* m)
+{
+    std::strncpy(g_last_error, m, sizeof(g_last_error) - 1);
+    g_last_error[sizeof(g_last_error) - 1] = 0;
+}
 
 template <typename F>
 int guarded(F&& fn)
 {
     try {
         fn();
-        g_last_error.clear();
+        g_last_error[0] = 0;
         return SMIP_OK;
     } catch (const Error& e) {
-        g_last_error = e.what();
+        set_last_error(e.what());
         return e.code;
     } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
+        set_last_error("host allocation failed");
         return SMIP_ERR_NO_MEMORY;
     } catch (const std::exception& e) {
-        g_last_error = e.what();
+        set_last_error(e.what());
         return SMIP_ERR_RUNTIME;
     } catch (...) {
-        g_last_error = "unknown error";
+        set_last_error("unknown error");
         return SMIP_ERR_RUNTIME;
     }
 }
@@ -324,7 +334,7 @@ void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
 // ---------------------------------------------------------------------------
 // misc
 // ---------------------------------------------------------------------------
-extern "C" SMIP_API const char* smip_last_error(void) { return g_last_error.c_str(); }
+extern "C" SMIP_API const char* smip_last_error(void) { return g_last_error; }
 
 extern "C" SMIP_API int smip_version(int32_t* major, int32_t* minor, int32_t* patch)
 {

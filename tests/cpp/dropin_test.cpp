@@ -420,15 +420,21 @@ GPU_CASE(bispectrum_accumulate_from_fft)
 
 GPU_CASE(phase_recovery_stepwise)
 {
-    // reconstruct_phases equals the reference's spiral driven step by step
-    // through calc_phase (include/phasereco.h:33-73), bit for bit
+    // The reference's spiral (include/phasereco.h:33-73) driven step by step
+    // through calc_phase equals reconstruct_phases in its one-visit-at-a-time
+    // schedule bit for bit, and the windowed schedule (pre-window contributions
+    // summed in bulk, so another FP64 summation order) with identical flags
+    // and phases within 1e-10.
     const int N = 32;
     const double radius = 12.0;
     Bispectrum<cd> B(Bispectrum<cd>::extents { N, N, 24, 24 });
     for (unsigned f = 0; f < 3; ++f)
         B.accumulate_from_fft(dft2(random_image(N, N, 100 + f)));
-    PhaseMap pm_dev;
+    PhaseMap pm_dev, pm_seq;
     auto ph_dev = reconstruct_phases<cd>(B, N, N, radius, &pm_dev);
+    setenv("SMIP_RECO_SEQUENTIAL", "1", 1);
+    auto ph_seq = reconstruct_phases<cd>(B, N, N, radius, &pm_seq);
+    unsetenv("SMIP_RECO_SEQUENTIAL");
 
     Array2<cd> ph(N, N);
     PhaseMap pm(N, N);
@@ -452,19 +458,27 @@ GPU_CASE(phase_recovery_stepwise)
         }
     }
     CHECK(steps > 100);
-    int bad_ph = 0, bad_fl = 0, bad_co = 0, flagged = 0;
+    int bad_ph = 0, bad_fl = 0, bad_co = 0, flagged = 0, bad_fl_w = 0;
+    double dph = 0., dco = 0.;
     for (int y = 0; y < N; ++y)
         for (int x = 0; x < N; ++x) {
-            const cd a = ph(x, y), b = ph_dev(x, y);
+            const cd a = ph(x, y), b = ph_seq(x, y);
             bad_ph += std::memcmp(&a, &b, sizeof(cd)) != 0;
-            bad_fl += pm(x, y).flag != pm_dev(x, y).flag;
-            bad_co += pm(x, y).consistency != pm_dev(x, y).consistency;
+            bad_fl += pm(x, y).flag != pm_seq(x, y).flag;
+            bad_co += std::memcmp(&pm(x, y).consistency, &pm_seq(x, y).consistency, sizeof(double)) != 0;
             flagged += pm(x, y).flag;
+            bad_fl_w += pm(x, y).flag != pm_dev(x, y).flag;
+            dph = std::max(dph, std::abs(a - ph_dev(x, y)));
+            dco = std::max(dco, std::fabs(pm(x, y).consistency - pm_dev(x, y).consistency));
         }
-    std::cout << "  steps " << steps << ", flagged " << flagged << std::endl;
+    std::cout << "  steps " << steps << ", flagged " << flagged << ", windowed max |dphase| " << dph
+              << ", max |dconsistency| " << dco << std::endl;
     CHECK(bad_ph == 0);
     CHECK(bad_fl == 0);
     CHECK(bad_co == 0);
+    CHECK(bad_fl_w == 0);
+    CHECK(dph < 1e-10);
+    CHECK(dco < 1e-10);
 }
 
 double gauss(double x, double y, double cx, double cy, double s)
