@@ -5,6 +5,7 @@ Produces, next to the sources:
   speckle-stacker_amd/lib/libsmip_synth.so    host-only synthetic speckle frames
   speckle-stacker_amd/lib/smip_cli            smip-cli compatible driver
   speckle-stacker_amd/lib/smip_dropin_test    C++ drop-in surface test runner
+  speckle-stacker_amd/lib/smip_gif_dump       the CLI's GIF decoder, for the tests
   speckle-stacker_amd/lib/smip_copy_calib     copy kernel calibrating rocprofv3 byte counters
 
 Run ``python speckle-stacker_amd/build.py`` or call ``build()``.
@@ -81,12 +82,17 @@ def build_cli() -> list[str]:
     common = ["-std=c++20", "-O2", f"-I{INCLUDE}", f"-I{PKG}", "-Wl,-rpath,$ORIGIN", f"-L{LIB}", "-lsmip_mi355x"]
     if os.path.exists(cli):
         out = os.path.join(LIB, "smip_cli")
-        _run(["g++", cli, "-o", out, *common])
+        _run(["g++", cli, "-o", out, *common, "-lz"])
         built.append(out)
     test = os.path.join(REPO, "tests", "cpp", "dropin_test.cpp")
     if os.path.exists(test):
         out = os.path.join(LIB, "smip_dropin_test")
         _run(["g++", test, "-o", out, *common])
+        built.append(out)
+    dump = os.path.join(REPO, "tests", "cpp", "gif_dump.cpp")
+    if os.path.exists(dump):
+        out = os.path.join(LIB, "smip_gif_dump")
+        _run(["g++", dump, "-o", out, "-std=c++20", "-O2", f"-I{INCLUDE}", f"-I{PKG}"])
         built.append(out)
     return built
 

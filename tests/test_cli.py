@@ -1,0 +1,296 @@
+"""smip_cli: the reference CLI's flags and outputs (smip-cli/main.cpp:40-401) on
+the engine.
+
+CPU tests cover option handling and the GIF decoder (against PIL, an
+independent decoder).  GPU tests run the CLI end to end on the reference's own
+example input (tests/golden/hu940ani.gif, "-b 32 -p 64 -c r") and on a .npy
+stack, and check every output against the oracle's pipeline on the same decoded
+frames:
+  * bispectrum.dat: header (base_size, constructor dims) exact; complex<float>
+    cells within 1e-5 of max|B| (mixed) / 1e-6 (fp64) of the FP64 oracle;
+  * sum_image, reco_image: 16-bit samples within 1 LSB of the oracle's values
+    mapped the way the reference maps them (include/videoio.h:100-211);
+  * phases: within 1 LSB on all but 0.5 % of pixels (the arg() branch cut at
+    phase -1 and the sign of zero where the window vanishes can flip a pixel);
+  * phasecons (heat map): within 1 LSB per channel;
+  * powerspec: pixels below the wrap point within 1 LSB (fp64) / 16 LSB (mixed:
+    FP32 transform, scaled by 65.5e6 at the spectrum's 1/1000 level).
+"""
+import json
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "speckle-stacker_amd", "lib")
+CLI = os.path.join(LIB, "smip_cli")
+DUMP = os.path.join(LIB, "smip_gif_dump")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+GIF = os.path.join(GOLDEN, "hu940ani.gif")
+OUTPUTS = ["sum_image_falsecolor.png", "sum_image.png", "phases_falsecolor.png", "phases.png", "phasecons.png",
+           "powerspec_falsecolor.png", "powerspec.png", "reco_image_falsecolor.png", "reco_image.png"]
+
+
+def _tool(path):
+    if not os.path.exists(path):
+        import sys
+        sys.path.insert(0, os.path.join(REPO, "speckle-stacker_amd"))
+        import build
+        build.build_cli()
+    return path
+
+
+def _cli(args, cwd, timeout=300):
+    return subprocess.run([_tool(CLI), *args], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+# ------------------------------------------------------------------ helpers
+def read_png16(path):
+    """Our own 16-bit PNGs (filter 0 on every row)."""
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n, typ = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        if typ == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, depth, ctype = hdr[:4]
+    assert depth == 16 and ctype in (0, 2)
+    ch = 3 if ctype == 2 else 1
+    raw = np.frombuffer(zlib.decompress(idat), dtype=np.uint8).reshape(h, 1 + 2 * w * ch)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].copy().view(">u2").astype(np.int64).reshape(h, w, ch).squeeze()
+
+
+def grey16(x):
+    v = np.asarray(x, dtype=np.float64) * 65535.0
+    out = np.zeros(v.shape, dtype=np.int64)
+    ok = np.abs(v) < 2147483648.0
+    out[ok] = np.trunc(v[ok]).astype(np.int64) & 0xFFFF
+    return out
+
+
+def heat16(x):
+    v = np.clip(np.asarray(x, dtype=np.float64), 0.0, 1.0)
+    m = 65535.0
+    r = np.where(v < 0.5, 0.0, np.where(v < 0.75, m * (4 * v - 2), m))
+    g = np.where(v < 0.25, 0.0, np.where(v < 0.5, m * (4 * v - 1), np.where(v < 0.75, m * (3 - 4 * v), 0.0)))
+    b = np.where(v < 0.25, m * 4 * v, np.where(v < 0.5, m * (2 - 4 * v), np.where(v < 0.75, 0.0, m * (4 * v - 3))))
+    return np.stack([np.trunc(r), np.trunc(g), np.trunc(b)], axis=-1).astype(np.int64)
+
+
+def centred(a):
+    """Signed symmetry of Array2Mat: pixel (j, i) shows index (j - X/2, i - Y/2)."""
+    return np.roll(np.roll(a, a.shape[0] // 2, axis=0), a.shape[1] // 2, axis=1)
+
+
+def hann(X, Y, aperture):
+    def axis(n):
+        i = np.arange(n)
+        s = np.where(i >= n - n // 2, i - n, i).astype(np.float64)
+        return 0.5 + 0.5 * np.cos(2 * np.pi / aperture * s)
+    return np.outer(axis(Y), axis(X))
+
+
+def gif_frames_pil(path):
+    from PIL import Image, ImageSequence
+    im = Image.open(path)
+    return np.stack([np.asarray(f.convert("RGB")) for f in ImageSequence.Iterator(im)])
+
+
+def gif_frames_ours(path, tmp_path):
+    out = os.path.join(tmp_path, "frames.rgb")
+    r = subprocess.run([_tool(DUMP), path, out], capture_output=True, text=True, check=True)
+    n, X, Y = map(int, r.stdout.split())
+    return np.fromfile(out, dtype=np.uint8).reshape(n, Y, X, 3)
+
+
+# ------------------------------------------------------------------ CPU
+def test_version(tmp_path):
+    r = _cli(["--version"], tmp_path)
+    assert r.returncode == 0 and r.stdout.startswith("Speckle Masking Image Processing v")
+
+
+@pytest.mark.parametrize("args", [["-h"], ["-?"], ["--help"], [], ["-f", "x.gif"], ["a.gif", "b.gif"]])
+def test_usage_exits_zero(tmp_path, args):
+    r = _cli(args, tmp_path)
+    assert r.returncode == 0 and "Usage" in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [(["-k", "-1:0"], "invalid crop box pos arguments"),
+                                      (["-s", "0:10"], "invalid crop box size arguments")])
+def test_bad_crop_box(tmp_path, args, msg):
+    r = _cli([*args, GIF], tmp_path)
+    assert r.returncode != 0 and msg in r.stderr
+
+
+def test_missing_input(tmp_path):
+    r = _cli([os.path.join(tmp_path, "nothing.gif")], tmp_path)
+    assert r.returncode != 0 and "file open error" in r.stderr
+
+
+def test_gif_decoder_matches_pil_on_reference_data(tmp_path):
+    ours = gif_frames_ours(GIF, tmp_path)
+    assert ours.shape == (40, 128, 128, 3)  # tests/opencv_test.cpp:18
+    assert np.array_equal(ours, gif_frames_pil(GIF))
+
+
+def _lzw_literal(indices):
+    """A valid GIF LZW stream of literal codes (min code size 8): a clear code
+    every 254 literals keeps the code width at 9 bits."""
+    codes, out, acc, nbits = [], bytearray(), 0, 0
+    for k, v in enumerate(indices):
+        if k % 254 == 0:
+            codes.append(256)
+        codes.append(int(v))
+    codes.append(257)
+    for c in codes:
+        acc |= c << nbits
+        nbits += 9
+        while nbits >= 8:
+            out.append(acc & 0xFF)
+            acc >>= 8
+            nbits -= 8
+    if nbits:
+        out.append(acc & 0xFF)
+    return bytes(out)
+
+
+def _sub_blocks(data):
+    out = bytearray()
+    for i in range(0, len(data), 255):
+        out += bytes([len(data[i:i + 255])]) + data[i:i + 255]
+    return bytes(out) + b"\x00"
+
+
+def write_gif(path, W, H, gct, frames):
+    """frames: dicts with rect (l, t, w, h), indices (h, w), and optional lct,
+    transparent index, disposal method, interlace."""
+    b = bytearray(b"GIF89a" + struct.pack("<HHBBB", W, H, 0xF7, 0, 0) + gct.astype(np.uint8).tobytes())
+    for f in frames:
+        tr = f.get("transparent")
+        b += b"\x21\xF9\x04" + bytes([(f.get("disposal", 0) << 2) | (1 if tr is not None else 0)])
+        b += struct.pack("<H", 4) + bytes([tr or 0, 0])
+        l, t, w, h = f["rect"]
+        packed = (0x80 | 7 if f.get("lct") is not None else 0) | (0x40 if f.get("interlace") else 0)
+        b += b"\x2C" + struct.pack("<HHHHB", l, t, w, h, packed)
+        if f.get("lct") is not None:
+            b += f["lct"].astype(np.uint8).tobytes()
+        rows = np.arange(h)
+        if f.get("interlace"):
+            rows = np.concatenate([np.arange(0, h, 8), np.arange(4, h, 8), np.arange(2, h, 4), np.arange(1, h, 2)])
+        b += b"\x08" + _sub_blocks(_lzw_literal(f["indices"][rows].ravel()))
+    b += b"\x3B"
+    open(path, "wb").write(bytes(b))
+
+
+def test_gif_decoder_partial_transparent_interlaced_frames(tmp_path):
+    """Sub-rectangle frames, transparency, local palettes, interlacing and the
+    keep (1) / restore-previous (3) disposals, against PIL."""
+    rng = np.random.default_rng(5)
+    W, H = 40, 27
+    gct = rng.integers(0, 256, (256, 3))
+    lct = rng.integers(0, 256, (256, 3))
+    frames = [
+        dict(rect=(0, 0, W, H), indices=rng.integers(0, 256, (H, W)), disposal=1),
+        dict(rect=(5, 3, 17, 11), indices=rng.integers(0, 4, (11, 17)), transparent=2, disposal=1),
+        dict(rect=(20, 9, 19, 18), indices=rng.integers(0, 256, (18, 19)), lct=lct, interlace=True, disposal=3),
+        dict(rect=(1, 1, 30, 13), indices=rng.integers(0, 256, (13, 30)), interlace=True, transparent=7,
+             disposal=1),
+        dict(rect=(0, 0, W, H), indices=rng.integers(250, 256, (H, W)), transparent=255, lct=lct),
+    ]
+    path = os.path.join(tmp_path, "t.gif")
+    write_gif(path, W, H, gct, frames)
+    ours = gif_frames_ours(path, tmp_path)
+    assert ours.shape == (5, H, W, 3)
+    assert np.array_equal(ours, gif_frames_pil(path))
+
+
+# ------------------------------------------------------------------ GPU
+def _oracle_pipeline(frames, b, p):
+    import oracle as O
+    return O.pipeline(frames.astype(np.float64), b, p)
+
+
+def _check_outputs(out_dir, want, b, p, precision):
+    N = want.image.shape[0]
+    for name in OUTPUTS + ["bispectrum.dat"]:
+        assert os.path.exists(os.path.join(out_dir, name)), name
+    # bispectrum.dat: size_t base_size, 4 x size_t constructor dims, complex<float> cells
+    raw = open(os.path.join(out_dir, "bispectrum.dat"), "rb").read()
+    base_size, d0, d1, d2, d3 = struct.unpack("<5Q", raw[:40])
+    assert (d0, d1, d2, d3) == (N, N, b, b)
+    assert base_size == want.bispectrum.size and len(raw) == 40 + 8 * base_size
+    B = np.frombuffer(raw[40:], dtype=np.complex64)
+    tol = (1e-5 if precision == "mixed" else 1e-6) * np.abs(want.bispectrum).max()
+    assert np.abs(B - want.bispectrum).max() <= tol
+
+    def lsb(a, e):
+        d = np.abs(a - e)
+        return np.minimum(d, 65536 - d)
+
+    assert lsb(read_png16(os.path.join(out_dir, "sum_image.png")), grey16(np.abs(want.sum_image / 255))).max() <= 1
+    img = centred(np.abs(want.image))
+    assert lsb(read_png16(os.path.join(out_dir, "reco_image.png")), grey16(img)).max() <= 1
+    assert lsb(read_png16(os.path.join(out_dir, "reco_image_falsecolor.png")), heat16(img)).max() <= 1
+    wph = want.phases * hann(N, N, 2.0 * p).astype(np.complex128)
+    ph = centred((np.angle(wph) + np.pi) / (2 * np.pi))
+    bad = lsb(read_png16(os.path.join(out_dir, "phases.png")), grey16(ph)) > 1
+    assert bad.mean() <= 0.005
+    assert lsb(read_png16(os.path.join(out_dir, "phasecons.png")), heat16(centred(want.consistency))).max() <= 1
+    P = np.abs(want.powerspec)
+    x = centred(P / (P.max() / 1000.0))
+    got = read_png16(os.path.join(out_dir, "powerspec.png"))
+    below = x <= 1.0
+    assert lsb(got, grey16(x))[below].max() <= (16 if precision == "mixed" else 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_cli_reference_example(tmp_path, precision):
+    """smip-cli -v -b 32 -p 64 -c r hu940ani.gif (README.md:91)."""
+    r = _cli(["-v", "-b", "32", "-p", "64", "-c", "r", "--precision", precision, "--stats", GIF], tmp_path)
+    assert r.returncode == 0, r.stderr
+    stats = json.loads(r.stdout.strip().splitlines()[-1])
+    assert stats["frames"] == 40 and stats["bdepth"] == 32 and stats["reco_radius"] == 64
+    frames = gif_frames_pil(GIF)[..., 0]  # -c r: the red channel
+    want = _oracle_pipeline(frames, 32, 64)
+    _check_outputs(str(tmp_path), want, 32, 64, precision)
+    # qualitative agreement with the reference's own output for this example
+    from PIL import Image
+    ref = np.asarray(Image.open(os.path.join(GOLDEN, "hu940ani_reco_image.png"))).astype(np.float64)
+    ours = read_png16(os.path.join(tmp_path, "reco_image.png")).astype(np.float64)
+    assert np.unravel_index(ours.argmax(), ours.shape) == np.unravel_index(ref.argmax(), ref.shape) == (64, 64)
+    assert np.corrcoef(ours.ravel(), ref.ravel())[0, 1] > 0.98
+
+
+@pytest.mark.gpu
+def test_cli_defaults_and_npy_stack(tmp_path):
+    """A .npy uint8 stack with the reference's defaults (b = 20, p = 40) and -n."""
+    import smip_amd as S
+    frames = S.synth_frames(S.synth_params("binary128"), 0, 40)
+    np.save(os.path.join(tmp_path, "stack.npy"), frames)
+    r = _cli(["-n", "24", "--stats", "stack.npy"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    stats = json.loads(r.stdout.strip().splitlines()[-1])
+    assert stats["frames"] == 24 and stats["bdepth"] == 20 and stats["reco_radius"] == 40
+    _check_outputs(str(tmp_path), _oracle_pipeline(frames[:24], 20, 40), 20, 40, "mixed")
+
+
+@pytest.mark.gpu
+def test_cli_raw_uint16_stack(tmp_path):
+    """A headerless uint16 stack (--raw) equals the same frames given as .npy."""
+    import smip_amd as S
+    frames = S.synth_frames(S.synth_params("triple256"), 0, 16)[:, 64:192, 64:192].copy()
+    frames.tofile(os.path.join(tmp_path, "stack.u16"))
+    r = _cli(["-b", "16", "-p", "32", "--raw", "128x128:u16", "--outdir", str(tmp_path), "stack.u16"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    _check_outputs(str(tmp_path), _oracle_pipeline(frames, 16, 32), 16, 32, "mixed")
