@@ -103,6 +103,102 @@ def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int):
     return lat, stats
 
 
+def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, camera_hz: float = 1000.0) -> dict:
+    """The request loop under Poisson arrivals: jobs drawn as the multi-sequence
+    mix (frame side from {64, 128, 256, 512}, frame count log-uniform in
+    [40, 5000], b uniform in [16, 64], p = 2b, object and sample type per job),
+    each streaming its frames in chunks of `chunk` at `camera_hz` per job.
+    Frames cycle through a pool of pre-generated frames per (side, type)."""
+    import heapq
+
+    rng = np.random.default_rng(seed)
+    kinds = ["binary128", "triple256", "extended512"]
+    pools = {}
+    for N in (64, 128, 256, 512):
+        for kind in kinds:
+            prm = S.synth_params(kind)
+            prm.ncols = prm.nrows = N
+            prm.pupil_diameter = max(8.0, N / 4)
+            pools[(N, kind)] = S.synth_frames(prm, 0, 64, threads=min(16, os.cpu_count() or 1))
+    # arrivals
+    jobs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        N = int(rng.choice([64, 128, 256, 512]))
+        kind = kinds[int(rng.integers(0, 3))]
+        n = int(round(np.exp(rng.uniform(np.log(40), np.log(5000)))))
+        b = int(rng.integers(16, 65))
+        jobs.append(dict(t=t, N=N, kind=kind, n=n, b=b, p=2 * b, off=int(rng.integers(0, 64))))
+    srv = S.Server(ctx)
+    pending = []  # (time, seq, job index, first frame)
+    for i, j in enumerate(jobs):
+        heapq.heappush(pending, (j["t"], i, i, 0))
+    closed = []
+    done_lock = threading.Lock()
+    results = {}
+    stop = threading.Event()
+
+    def reaper():
+        k = 0
+        while not (stop.is_set() and k >= len(closed)):
+            if k >= len(closed):
+                time.sleep(0.005)
+                continue
+            i, jid = closed[k]
+            srv.wait(jid)
+            info = srv.info(jid)
+            srv.release(jid)
+            with done_lock:
+                results[i] = info
+            k += 1
+
+    th = threading.Thread(target=reaper)
+    th.start()
+    t0 = time.perf_counter()
+    frames_pushed = 0
+    seq = len(jobs)
+    while pending:
+        when, _, i, first = heapq.heappop(pending)
+        delay = t0 + when - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        j = jobs[i]
+        pool = pools[(j["N"], j["kind"])]
+        if first == 0:
+            j["id"] = srv.submit(S.job_params(j["N"], j["N"], j["b"], j["p"], pool.dtype))
+        cnt = min(chunk, j["n"] - first)
+        idx = (np.arange(first, first + cnt) + j["off"]) % len(pool)
+        srv.push(j["id"], pool[idx])
+        frames_pushed += cnt
+        if first + cnt >= j["n"]:
+            srv.close_job(j["id"])
+            closed.append((i, j["id"]))
+        else:
+            seq += 1
+            heapq.heappush(pending, (when + cnt / camera_hz, seq, i, first + cnt))
+    stop.set()
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([results[i]["latency_ms"] for i in range(len(jobs))])
+    tail = np.array([results[i]["tail_ms"] for i in range(len(jobs))])
+    queue = np.array([results[i]["queue_ms"] for i in range(len(jobs))])
+    pct = lambda a, q: round(float(np.percentile(a, q)), 2)
+    return {"arrival": f"Poisson, {rate} jobs/s for {seconds} s", "jobs": len(jobs), "completed": st["completed"],
+            "failed": st["failed"], "frames": int(frames_pushed), "wall_s": round(wall, 2),
+            "frames_per_s": round(frames_pushed / wall, 1),
+            "latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
+            "after_last_frame_ms": {"p50": pct(tail, 50), "p95": pct(tail, 95), "p99": pct(tail, 99)},
+            "admission_wait_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)},
+            "camera": f"chunks of {chunk} frames at {camera_hz:.0f} frames/s per job",
+            "mix": "side {64,128,256,512}, frames log-uniform 40-5000, b U[16,64], p=2b, "
+                   "binary/u8, triple/u16, extended/f32"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +213,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson job arrivals per second")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -227,6 +325,13 @@ def main() -> int:
             "shifts_equal": bool(np.array_equal(rm.shifts, rd.shifts)),
         }
         log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
+
+    if args.serve_seconds > 0:
+        log(f"serving: Poisson arrivals at {args.serve_rate} jobs/s for {args.serve_seconds} s")
+        out["serving"] = serve_leg(ctx, args.serve_seconds, args.serve_rate, seed=0x5EC1E005)
+        sv = out["serving"]
+        log(f"serving: {sv['completed']}/{sv['jobs']} jobs, {sv['frames_per_s']} frames/s, latency {sv['latency_ms']}, "
+            f"after last frame {sv['after_last_frame_ms']}")
 
     if not args.no_cpu:
         log("timing the CPU baseline")

@@ -214,6 +214,14 @@ SMIP_API int smip_job_destroy(smip_job* j);
 SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, uint64_t first, uint64_t count);
 /* Upload host frames and enqueue them. */
 SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t count);
+/* Streaming form of the frame loop (smip-cli/main.cpp:281-306): appends count
+ * host frames to the job's current batch and processes a batch whenever it is
+ * full; smip_job_finalize processes the last, partial batch.  Batch boundaries
+ * depend only on the frame sequence, not on how it is split into calls, so the
+ * result is the same however the frames arrive.  The host buffer may be reused
+ * on return.  Not to be mixed with smip_job_add_frames(_host) while pushed
+ * frames are pending. */
+SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frames, uint64_t count);
 /* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376). */
 SMIP_API int smip_job_finalize(smip_job* j);
 SMIP_API int smip_job_synchronize(smip_job* j);
@@ -227,6 +235,66 @@ SMIP_API int smip_job_get_sum_image(smip_job* j, double* out);        /* SAA sum
 SMIP_API int smip_job_get_shifts(smip_job* j, int32_t* out, uint64_t count); /* count x 2 */
 SMIP_API int smip_job_get_phases(smip_job* j, double* phases, uint8_t* flags, double* consistency);
 SMIP_API int smip_job_get_image(smip_job* j, double* out);            /* complex, /max|.| */
+
+/* ------------------------------------------------------------------------ */
+/* Serving: a continuous-batching request loop (no counterpart in the        */
+/* reference, which processes one stack per process, smip-cli/main.cpp)      */
+/* ------------------------------------------------------------------------ */
+/* Jobs are submitted, stream frames in chunks and are closed; they are
+ * admitted first-fit in arrival order when a slot is free and their
+ * smip_job_footprint fits the HBM budget beside the admitted jobs, keep their
+ * state on the device while frames arrive, and overlap on the GPU (one stream
+ * per job, worker threads drive the chunks).  A job's result is the same alone
+ * and in any mix: batch boundaries follow its own frame sequence only.  Device
+ * memory returns to the budget when the job is released. */
+typedef struct smip_server smip_server;
+
+typedef struct smip_server_params {
+    uint64_t hbm_budget; /* device bytes admitted jobs may hold; 0 = 90% of free memory */
+    int32_t max_active;  /* jobs admitted at once; 0 = 16 */
+    int32_t workers;     /* host threads driving admitted jobs; 0 = 8 */
+} smip_server_params;
+
+#define SMIP_JOB_WAITING 0 /* submitted, not admitted */
+#define SMIP_JOB_ACTIVE 1  /* admitted, frames streaming or being processed */
+#define SMIP_JOB_DONE 2    /* finalised; results readable through smip_server_job */
+#define SMIP_JOB_FAILED 3
+
+typedef struct smip_server_job_info {
+    int32_t state;
+    uint64_t footprint;               /* device bytes */
+    uint64_t frames_in, frames_done;  /* pushed / processed */
+    double queue_ms;                  /* submit -> admission (or now) */
+    double latency_ms;                /* submit -> completion (or now) */
+    double tail_ms;                   /* close -> completion (or now) */
+    char error[256];
+} smip_server_job_info;
+
+typedef struct smip_server_stats {
+    uint64_t submitted, admitted, completed, failed;
+    uint64_t waiting, active;       /* now */
+    uint64_t bytes_in_use, budget;  /* admitted footprints / budget */
+    uint64_t frames;                /* frames processed */
+} smip_server_stats;
+
+SMIP_API int smip_server_create(smip_ctx* ctx, const smip_server_params* p, smip_server** out);
+/* Stops the workers and frees every job, finished or not. */
+SMIP_API int smip_server_destroy(smip_server* s);
+/* Footprint above the budget -> SMIP_ERR_NO_MEMORY; bad parameters -> SMIP_ERR_INVALID_ARGUMENT. */
+SMIP_API int smip_server_submit(smip_server* s, const smip_job_params* p, uint64_t* id);
+/* Copies count frames (the job's dtype and extents); the buffer may be reused on return. */
+SMIP_API int smip_server_push(smip_server* s, uint64_t id, const void* frames, uint64_t count);
+/* No more frames: the job is finalised once its frames are processed. */
+SMIP_API int smip_server_close(smip_server* s, uint64_t id);
+/* Blocks until the job is done or failed (timeout_s < 0: no limit); state may be NULL.
+ * A failed job -> SMIP_ERR_RUNTIME with its error text. */
+SMIP_API int smip_server_wait(smip_server* s, uint64_t id, double timeout_s, int32_t* state);
+/* The finished job (borrowed, valid until release): read results with smip_job_get_*. */
+SMIP_API int smip_server_job(smip_server* s, uint64_t id, smip_job** out);
+SMIP_API int smip_server_get_job_info(smip_server* s, uint64_t id, smip_server_job_info* out);
+/* Frees the job's device state and returns its footprint to the budget. */
+SMIP_API int smip_server_release(smip_server* s, uint64_t id);
+SMIP_API int smip_server_get_stats(smip_server* s, smip_server_stats* out);
 
 #ifdef __cplusplus
 }

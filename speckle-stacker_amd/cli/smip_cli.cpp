@@ -317,7 +317,7 @@ int main(int argc, char* argv[])
 
         notice() << "writing bispectrum to file 'bispectrum.dat'";
         {
-            smip_bispec* b64 = nullptr;
+            smip_bispec* b64 = nullptr; // the job's own accumulator, borrowed
             smip_bispec* b32 = nullptr;
             check(smip_job_bispectrum(job, &b64), "bispectrum");
             const uint64_t dims[4] = { X, Y, bispectrum_depth, bispectrum_depth };
@@ -325,9 +325,12 @@ int main(int argc, char* argv[])
             int rc = smip_bispec_assign(b32, b64);
             if (rc == SMIP_OK)
                 rc = smip_bispec_write_file(b32, (outdir + "/bispectrum.dat").c_str());
+            if (rc != SMIP_OK) {
+                const std::string err = smip_last_error();
+                smip_bispec_destroy(b32);
+                throw std::runtime_error("bispectrum.dat: " + err);
+            }
             smip_bispec_destroy(b32);
-            smip_bispec_destroy(b64);
-            check(rc, "bispectrum.dat");
         }
 
         // phases *= Hann(X, Y, 2p) for the phase images (smip-cli/main.cpp:328-330)

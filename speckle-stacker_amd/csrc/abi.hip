@@ -17,41 +17,23 @@
 
 using namespace smip_amd;
 
-namespace {
-
+namespace smip_amd {
 // Trivially destructible on purpose: callers may reach the ABI from static
 // destructors, which run after the calling thread's thread_local objects have
 // been destroyed; a std::string here would then be written after its heap
 // buffer was freed.
-thread_local char g_last_error[512];
+static thread_local char g_last_error[512];
 
 void set_last_error(const char* m)
 {
     std::strncpy(g_last_error, m, sizeof(g_last_error) - 1);
     g_last_error[sizeof(g_last_error) - 1] = 0;
 }
+void clear_last_error() { g_last_error[0] = 0; }
+const char* last_error_text() { return g_last_error; }
+} // namespace smip_amd
 
-template <typename F>
-int guarded(F&& fn)
-{
-    try {
-        fn();
-        g_last_error[0] = 0;
-        return SMIP_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return SMIP_ERR_NO_MEMORY;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return SMIP_ERR_RUNTIME;
-    } catch (...) {
-        set_last_error("unknown error");
-        return SMIP_ERR_RUNTIME;
-    }
-}
+namespace {
 
 void require(bool cond, int code, const char* msg)
 {
@@ -161,6 +143,13 @@ struct smip_job {
     uint64_t shifts_cap = 0;
     // host upload staging
     dptr<uint8_t> upload;
+    // buffered streaming (smip_job_push_frames_host): frames collect in `upload`
+    // until a full batch; host data passes through two pinned slots
+    uint64_t staged = 0;
+    uint8_t* pin[2] = { nullptr, nullptr };
+    hipEvent_t pin_done[2] = { nullptr, nullptr };
+    uint64_t pin_frames = 0; // frames per pinned slot
+    int pin_cur = 0;
     // outputs
     dptr<double2> phases, image;
     dptr<uint8_t> flags;
@@ -179,6 +168,12 @@ struct smip_job {
         for (auto& e : events) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (pin_done[k])
+                (void)hipEventDestroy(pin_done[k]);
+            if (pin[k])
+                (void)hipHostFree(pin[k]);
         }
         tile_plan_destroy(tile);
         fft_plan_destroy(plan);
@@ -334,7 +329,7 @@ void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
 // ---------------------------------------------------------------------------
 // misc
 // ---------------------------------------------------------------------------
-extern "C" SMIP_API const char* smip_last_error(void) { return g_last_error; }
+extern "C" SMIP_API const char* smip_last_error(void) { return last_error_text(); }
 
 extern "C" SMIP_API int smip_version(int32_t* major, int32_t* minor, int32_t* patch)
 {
@@ -1193,6 +1188,7 @@ extern "C" SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, u
     return guarded([&] {
         require(j && f, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
         require(f->dtype == j->p.frame_dtype && f->X == j->p.ncols && f->Y == j->p.nrows, SMIP_ERR_INVALID_ARGUMENT,
             "frame stack does not match the job");
         require(first + count <= f->count, SMIP_ERR_OUT_OF_RANGE, "frame range outside the stack");
@@ -1208,6 +1204,7 @@ extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames
     return guarded([&] {
         require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
         ensure_device(j->ctx);
         grow_shifts(j, j->nframes + count);
         const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
@@ -1225,13 +1222,59 @@ extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames
     });
 }
 
+extern "C" SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frames, uint64_t count)
+{
+    return guarded([&] {
+        require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        ensure_device(j->ctx);
+        const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
+        const uint64_t batch = (uint64_t)j->batch;
+        if (!j->upload)
+            j->upload = dmake<uint8_t>((size_t)batch * fb);
+        if (!j->pin[0]) {
+            // pinned slots of at most 8 MiB (at least one frame), never more than a batch
+            j->pin_frames = std::max<uint64_t>(1, std::min<uint64_t>(batch, (8u << 20) / fb));
+            for (int k = 0; k < 2; ++k) {
+                SMIP_HIP(hipHostMalloc((void**)&j->pin[k], j->pin_frames * fb, hipHostMallocDefault));
+                SMIP_HIP(hipEventCreateWithFlags(&j->pin_done[k], hipEventDisableTiming));
+                SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
+            }
+        }
+        grow_shifts(j, j->nframes + j->staged + count);
+        const uint8_t* src = static_cast<const uint8_t*>(frames);
+        while (count > 0) {
+            const uint64_t n = std::min({ count, j->pin_frames, batch - j->staged });
+            const int k = j->pin_cur;
+            SMIP_HIP(hipEventSynchronize(j->pin_done[k])); // the slot's previous copy has left it
+            std::memcpy(j->pin[k], src, n * fb);
+            // stream order puts this copy after the kernels that read the previous batch
+            SMIP_HIP(hipMemcpyAsync(j->upload.get() + j->staged * fb, j->pin[k], n * fb, hipMemcpyHostToDevice,
+                j->stream));
+            SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
+            j->pin_cur ^= 1;
+            j->staged += n;
+            src += n * fb;
+            count -= n;
+            if (j->staged == batch) {
+                job_process(j, j->upload.get(), batch);
+                j->staged = 0;
+            }
+        }
+    });
+}
+
 extern "C" SMIP_API int smip_job_finalize(smip_job* j)
 {
     return guarded([&] {
         require(j != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
-        require(j->nframes > 0, SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
         ensure_device(j->ctx);
+        if (j->staged > 0) { // the last, partial batch of pushed frames
+            job_process(j, j->upload.get(), j->staged);
+            j->staged = 0;
+        }
+        require(j->nframes > 0, SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
         hipStream_t s = j->stream;
         const smip_job_params& p = j->p;
         // bispectrum /= complex(n, 0)

@@ -27,6 +27,34 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
 
+// the calling thread's last error text (smip_last_error)
+void set_last_error(const char* msg);
+void clear_last_error();
+const char* last_error_text();
+
+// runs fn, mapping exceptions to status codes and the last-error text
+template <typename F>
+int guarded(F&& fn)
+{
+    try {
+        fn();
+        clear_last_error();
+        return SMIP_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return SMIP_ERR_NO_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SMIP_ERR_RUNTIME;
+    } catch (...) {
+        set_last_error("unknown error");
+        return SMIP_ERR_RUNTIME;
+    }
+}
+
 inline void hip_check(hipError_t e, const char* what)
 {
     if (e != hipSuccess)

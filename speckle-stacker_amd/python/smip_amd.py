@@ -63,6 +63,24 @@ class JobStats(C.Structure):
                 ("triples_per_frame", C.c_uint64), ("tile_ms", C.c_double), ("tile_launches", C.c_uint64)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("workers", C.c_int32)]
+
+
+class ServerJobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("footprint", C.c_uint64), ("frames_in", C.c_uint64),
+                ("frames_done", C.c_uint64), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("tail_ms", C.c_double), ("error", C.c_char * 256)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("waiting", C.c_uint64), ("active", C.c_uint64),
+                ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64), ("frames", C.c_uint64)]
+
+
+JOB_WAITING, JOB_ACTIVE, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
 # name -> (restype, argtypes); every function in include/smip_mi355x.h
 _P = C.c_void_p
 _SIGS = {
@@ -105,6 +123,7 @@ _SIGS = {
     "smip_job_destroy": (C.c_int, [_P]),
     "smip_job_add_frames": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
     "smip_job_add_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
+    "smip_job_push_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_finalize": (C.c_int, [_P]),
     "smip_job_synchronize": (C.c_int, [_P]),
     "smip_job_get_stats": (C.c_int, [_P, _P]),
@@ -114,6 +133,16 @@ _SIGS = {
     "smip_job_get_shifts": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_get_phases": (C.c_int, [_P, _P, _P, _P]),
     "smip_job_get_image": (C.c_int, [_P, _P]),
+    "smip_server_create": (C.c_int, [_P, _P, _P]),
+    "smip_server_destroy": (C.c_int, [_P]),
+    "smip_server_submit": (C.c_int, [_P, _P, _P]),
+    "smip_server_push": (C.c_int, [_P, C.c_uint64, _P, C.c_uint64]),
+    "smip_server_close": (C.c_int, [_P, C.c_uint64]),
+    "smip_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, _P]),
+    "smip_server_job": (C.c_int, [_P, C.c_uint64, _P]),
+    "smip_server_get_job_info": (C.c_int, [_P, C.c_uint64, _P]),
+    "smip_server_release": (C.c_int, [_P, C.c_uint64]),
+    "smip_server_get_stats": (C.c_int, [_P, _P]),
 }
 
 _LIB: C.CDLL | None = None
@@ -400,17 +429,20 @@ def job_footprint(params: JobParams) -> int:
 class Job:
     """One speckle-masking reconstruction (smip-cli/main.cpp:242-376)."""
 
-    def __init__(self, ctx: Context, params: JobParams):
-        h = C.c_void_p()
-        _check(lib().smip_job_create(ctx.h, C.byref(params), C.byref(h)))
-        self.h = h
+    def __init__(self, ctx: Context, params: JobParams, _handle=None):
+        self.borrowed = _handle is not None
+        if _handle is None:
+            h = C.c_void_p()
+            _check(lib().smip_job_create(ctx.h, C.byref(params), C.byref(h)))
+            _handle = h
+        self.h = _handle
         self.ctx = ctx
         self.p = params
 
     def close(self):
-        if self.h:
+        if self.h and not self.borrowed:
             lib().smip_job_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -426,6 +458,11 @@ class Job:
     def add_frames_host(self, frames: np.ndarray) -> None:
         a = np.ascontiguousarray(frames)
         _check(lib().smip_job_add_frames_host(self.h, _p(a), a.shape[0]))
+
+    def push_frames_host(self, frames: np.ndarray) -> None:
+        """Streaming form: batch boundaries follow the frame sequence, not the calls."""
+        a = np.ascontiguousarray(frames)
+        _check(lib().smip_job_push_frames_host(self.h, _p(a), a.shape[0]))
 
     def finalize(self) -> None:
         _check(lib().smip_job_finalize(self.h))
@@ -478,6 +515,70 @@ class Job:
                          self.sum_image() if self.p.saa else None,
                          self.shifts(int(st["frames"])) if self.p.saa else None,
                          ph, fl, co, self.image(), st)
+
+
+class Server:
+    """The continuous-batching request loop (smip_server_*): jobs are admitted by
+    device footprint as slots and memory free up, stream frames in chunks and
+    overlap on the GPU."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, workers: int = 0):
+        h = C.c_void_p()
+        prm = ServerParams(hbm_budget, max_active, workers)
+        _check(lib().smip_server_create(ctx.h, C.byref(prm), C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.params: dict[int, JobParams] = {}
+
+    def close(self):
+        if self.h:
+            lib().smip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, params: JobParams) -> int:
+        jid = C.c_uint64()
+        _check(lib().smip_server_submit(self.h, C.byref(params), C.byref(jid)))
+        self.params[jid.value] = params
+        return jid.value
+
+    def push(self, jid: int, frames: np.ndarray) -> None:
+        a = np.ascontiguousarray(frames)
+        _check(lib().smip_server_push(self.h, jid, _p(a), a.shape[0]))
+
+    def close_job(self, jid: int) -> None:
+        _check(lib().smip_server_close(self.h, jid))
+
+    def wait(self, jid: int, timeout: float = -1.0) -> int:
+        st = C.c_int32()
+        _check(lib().smip_server_wait(self.h, jid, timeout, C.byref(st)))
+        return st.value
+
+    def job(self, jid: int) -> Job:
+        h = C.c_void_p()
+        _check(lib().smip_server_job(self.h, jid, C.byref(h)))
+        return Job(self.ctx, self.params[jid], _handle=h)
+
+    def info(self, jid: int) -> dict:
+        i = ServerJobInfo()
+        _check(lib().smip_server_get_job_info(self.h, jid, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in ServerJobInfo._fields_}
+        d["error"] = i.error.decode()
+        return d
+
+    def release(self, jid: int) -> None:
+        _check(lib().smip_server_release(self.h, jid))
+        self.params.pop(jid, None)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().smip_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
 
 
 def run_pipeline(ctx: Context, frames: np.ndarray, bdepth: int, reco_radius: float,

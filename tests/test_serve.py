@@ -1,0 +1,202 @@
+"""The continuous-batching request loop (smip_server_*) and the streaming frame
+entry point it drives (smip_job_push_frames_host).
+
+The property the serving form must keep: a job's result is the same alone and
+in any mix of concurrent jobs.  Batch boundaries follow a job's own frame
+sequence and every output cell is owned by one work item, so "the same" is
+checked bit for bit here, which is stronger than the engine's stated tolerance
+against the FP64 oracle (image rel L2 <= 1e-5, also checked)."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import smip_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(kind, N, n, seed_offset=0):
+    W = dict(S.WORKLOADS[kind])
+    prm = S.synth_params(kind)
+    prm.ncols = prm.nrows = N
+    prm.seed = W["seed"] + seed_offset
+    prm.pupil_diameter = max(8.0, N / 4)
+    return S.synth_frames(prm, 0, n, threads=8)
+
+
+def _outputs(job: S.Job):
+    r = job.result(with_bispectrum=True)
+    return dict(bispectrum=r.bispectrum, powerspec=r.powerspec, sum_image=r.sum_image, shifts=r.shifts,
+                phases=r.phases, flags=r.flags, consistency=r.consistency, image=r.image)
+
+
+def _assert_identical(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_push_is_independent_of_chunking(ctx):
+    frames = _frames("binary128", 64, 90)
+    p = S.job_params(64, 64, 16, 24, frames.dtype, frames_per_batch=32)
+    ref = S.Job(ctx, p)
+    ref.add_frames_host(frames)  # batches [0,32), [32,64), [64,90)
+    ref.finalize()
+    want = _outputs(ref)
+    ref.close()
+    for cuts in ([90], [1, 7, 33, 2, 47], [31, 1, 31, 1, 26]):
+        job = S.Job(ctx, p)
+        pos = 0
+        for c in cuts:
+            job.push_frames_host(frames[pos:pos + c])
+            pos += c
+        job.finalize()
+        _assert_identical(_outputs(job), want)
+        job.close()
+
+
+def test_push_mixing_rules(ctx):
+    frames = _frames("binary128", 64, 8)
+    job = S.Job(ctx, S.job_params(64, 64, 16, 24, frames.dtype, frames_per_batch=32))
+    job.push_frames_host(frames[:5])
+    with pytest.raises(S.SmipError) as e:
+        job.add_frames_host(frames[5:])
+    assert e.value.code == S.ERR_LOGIC
+    job.finalize()
+    with pytest.raises(S.SmipError):
+        job.push_frames_host(frames)
+    job.close()
+
+
+def test_served_job_matches_oracle(ctx):
+    frames = _frames("binary128", 128, 40)
+    srv = S.Server(ctx)
+    jid = srv.submit(S.job_params(128, 128, 32, 64, frames.dtype))
+    for k in range(0, 40, 9):
+        srv.push(jid, frames[k:k + 9])
+    srv.close_job(jid)
+    assert srv.wait(jid, 120.0) == S.JOB_DONE
+    got = srv.job(jid).result(with_bispectrum=False)
+    want = O.pipeline(frames.astype(np.float64), 32, 64)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    err = np.linalg.norm(got.image - want.image) / np.linalg.norm(want.image)
+    assert err < 1e-5, err
+    info = srv.info(jid)
+    assert info["frames_done"] == 40 and info["latency_ms"] >= info["tail_ms"] >= 0
+    srv.release(jid)
+    srv.close()
+
+
+# jobs of uneven frame size, count, dtype and bispectrum radius
+MIX = [("binary128", 64, 70, 16, 32), ("triple256", 128, 55, 24, 48), ("binary128", 128, 40, 32, 64),
+       ("extended512", 64, 33, 20, 40), ("triple256", 256, 24, 48, 96), ("binary128", 32, 100, 12, 16),
+       ("extended512", 128, 45, 16, 32)]
+
+
+def _run_alone(ctx, frames, p):
+    srv = S.Server(ctx, max_active=1, workers=1)
+    jid = srv.submit(p)
+    srv.push(jid, frames)
+    srv.close_job(jid)
+    assert srv.wait(jid, 120.0) == S.JOB_DONE
+    out = _outputs(srv.job(jid))
+    srv.release(jid)
+    srv.close()
+    return out
+
+
+def test_result_is_the_same_alone_and_in_the_mix(ctx):
+    data = []
+    for i, (kind, N, n, b, p) in enumerate(MIX):
+        fr = _frames(kind, N, n, seed_offset=100 + i)
+        prm = S.job_params(N, N, b, p, fr.dtype, frames_per_batch=16 + 8 * i)
+        data.append((fr, prm))
+    alone = [_run_alone(ctx, fr, prm) for fr, prm in data]
+
+    srv = S.Server(ctx, max_active=4, workers=4)
+    ids = [None] * len(data)
+    errors = []
+
+    def client(i):
+        try:
+            fr, prm = data[i]
+            ids[i] = srv.submit(prm)
+            rng = np.random.default_rng(i)
+            pos = 0
+            while pos < len(fr):
+                c = int(rng.integers(1, 23))
+                srv.push(ids[i], fr[pos:pos + c])
+                pos += c
+            srv.close_job(ids[i])
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=client, args=(i,)) for i in range(len(data))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i, jid in enumerate(ids):
+        assert srv.wait(jid, 300.0) == S.JOB_DONE, srv.info(jid)
+        _assert_identical(_outputs(srv.job(jid)), alone[i])
+        srv.release(jid)
+    st = srv.stats()
+    assert st["completed"] == len(data) and st["failed"] == 0 and st["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_admission_respects_budget_and_slots(ctx):
+    fr = _frames("binary128", 128, 20)
+    prm = S.job_params(128, 128, 32, 32, fr.dtype, frames_per_batch=16)
+    fp = S.job_footprint(prm)
+    srv = S.Server(ctx, hbm_budget=2 * fp + fp // 2, max_active=8, workers=4)
+    ids = [srv.submit(prm) for _ in range(5)]
+    st = srv.stats()
+    assert st["active"] == 2 and st["waiting"] == 3 and st["bytes_in_use"] == 2 * fp
+    for jid in ids:
+        srv.push(jid, fr)
+        srv.close_job(jid)
+    peak = 0
+    for jid in ids:
+        assert srv.wait(jid, 120.0) == S.JOB_DONE
+        peak = max(peak, srv.stats()["bytes_in_use"])
+        srv.release(jid)  # frees the slot for the next waiting job
+    assert peak <= 2 * fp
+    st = srv.stats()
+    assert st["completed"] == 5 and st["waiting"] == 0 and st["active"] == 0
+    # a job that can never fit is refused at submit
+    big = S.job_params(512, 512, 64, 128, np.uint16)
+    with pytest.raises(S.SmipError) as e:
+        srv.submit(big)
+    assert e.value.code == S.ERR_NO_MEMORY
+    srv.close()
+
+
+def test_server_errors(ctx):
+    srv = S.Server(ctx, max_active=2, workers=2)
+    fr = _frames("binary128", 64, 4)
+    prm = S.job_params(64, 64, 16, 16, fr.dtype)
+    jid = srv.submit(prm)
+    with pytest.raises(S.SmipError) as e:
+        srv.close_job(jid)  # no frames yet
+    assert e.value.code == S.ERR_RUNTIME
+    srv.push(jid, fr)
+    srv.close_job(jid)
+    with pytest.raises(S.SmipError) as e:
+        srv.push(jid, fr)
+    assert e.value.code == S.ERR_LOGIC
+    with pytest.raises(S.SmipError) as e:
+        srv.wait(jid + 1000)
+    assert e.value.code == S.ERR_INVALID_ARGUMENT
+    bad = S.job_params(48, 48, 16, 16, fr.dtype)  # not a power of two
+    with pytest.raises(S.SmipError):
+        srv.submit(bad)
+    assert srv.wait(jid, 60.0) == S.JOB_DONE
+    srv.release(jid)
+    with pytest.raises(S.SmipError):
+        srv.info(jid)
+    srv.close()
