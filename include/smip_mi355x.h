@@ -222,6 +222,10 @@ SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t 
  * on return.  Not to be mixed with smip_job_add_frames(_host) while pushed
  * frames are pending. */
 SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frames, uint64_t count);
+/* The parameters a job of p is created with: vy_tile and frames_per_batch 0
+ * replaced by the autotuner's cached choice for (ncols, nrows, bdepth) when
+ * there is one, else by the built-in defaults. */
+SMIP_API int smip_job_resolve_params(const smip_job_params* p, smip_job_params* out);
 /* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376). */
 SMIP_API int smip_job_finalize(smip_job* j);
 SMIP_API int smip_job_synchronize(smip_job* j);
@@ -235,6 +239,26 @@ SMIP_API int smip_job_get_sum_image(smip_job* j, double* out);        /* SAA sum
 SMIP_API int smip_job_get_shifts(smip_job* j, int32_t* out, uint64_t count); /* count x 2 */
 SMIP_API int smip_job_get_phases(smip_job* j, double* phases, uint8_t* flags, double* consistency);
 SMIP_API int smip_job_get_image(smip_job* j, double* out);            /* complex, /max|.| */
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner of the batched triple-product kernel (no counterpart in the     */
+/* reference)                                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct smip_tune_result {
+    int32_t vy_tile;          /* vy values per register tile */
+    int32_t frames_per_batch; /* FP32 frames per FP64 fold */
+    double frames_per_s;      /* triple-product kernel rate at that choice */
+    double tile_tflops;       /* the same in algorithmic TFLOP/s (14 per triple) */
+    int32_t candidates;       /* timed configurations */
+} smip_tune_result;
+/* Times the kernel on synthetic spectra for every vy tile, then for batch
+ * sizes 32..1024 that fit max_bytes (0 = a quarter of free memory) beside the
+ * bispectrum; picks the smallest batch within 3 % of the best frames/s and
+ * caches the choice for (ncols, nrows, bdepth) in this process. */
+SMIP_API int smip_autotune(smip_ctx* ctx, uint64_t ncols, uint64_t nrows, uint64_t bdepth, uint64_t max_bytes,
+    smip_tune_result* out);
+/* Clears the cache (jobs then use the built-in defaults again). */
+SMIP_API int smip_autotune_forget(void);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: a continuous-batching request loop (no counterpart in the        */

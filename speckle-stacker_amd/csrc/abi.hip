@@ -74,11 +74,6 @@ constexpr double k2pi = 3.14159265358979323846 * 2.0; // constants::c_2pi<double
 // ---------------------------------------------------------------------------
 // objects
 // ---------------------------------------------------------------------------
-struct smip_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
-
 struct smip_bispec {
     smip_ctx* ctx = nullptr;
     Desc d {};
@@ -322,7 +317,6 @@ __global__ void k_frame_to_f64(const void* __restrict__ in, int dtype, int64_t n
     out[i] = v;
 }
 
-void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
 
 } // namespace
 
@@ -945,8 +939,9 @@ JobSizes job_sizes(const smip_job_params& p)
     auto pow2 = [](uint64_t n) { return n >= 8 && n <= 2048 && (n & (n - 1)) == 0; };
     require(pow2(p.ncols) && pow2(p.nrows), SMIP_ERR_INVALID_ARGUMENT,
         "job frame extents must be powers of two in [8, 2048]");
-    require(p.bdepth >= 1 && p.bdepth / 2 < std::min(p.ncols, p.nrows) / 2, SMIP_ERR_INVALID_ARGUMENT,
-        "bispectrum depth must satisfy 1 <= b and b/2 < N/2");
+    // the accumulate loops clip v to the frame's range (include/bispectrum.h:383-391), so
+    // b may exceed N; phase recovery bounds the candidate window at b/2 <= 63
+    require(p.bdepth >= 1 && p.bdepth <= 127, SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth must satisfy 1 <= b <= 127");
     require(p.frame_dtype == SMIP_DTYPE_U8 || p.frame_dtype == SMIP_DTYPE_U16 || p.frame_dtype == SMIP_DTYPE_F32,
         SMIP_ERR_INVALID_ARGUMENT, "frame dtype must be u8, u16 or f32");
     require(p.precision == SMIP_PRECISION_FP64 || p.precision == SMIP_PRECISION_MIXED, SMIP_ERR_INVALID_ARGUMENT,
@@ -958,6 +953,10 @@ JobSizes job_sizes(const smip_job_params& p)
     z.L = loop_bounds(z.d, (int64_t)p.ncols, (int64_t)p.nrows);
     const bool mixed = p.precision == SMIP_PRECISION_MIXED;
     z.vy_tile = mixed ? (p.vy_tile > 0 ? p.vy_tile : 0) : 0;
+    int tuned_vy = 0, tuned_batch = 0;
+    const bool tuned = mixed && tune_lookup(p.ncols, p.nrows, p.bdepth, &tuned_vy, &tuned_batch);
+    if (mixed && z.vy_tile == 0 && tuned)
+        z.vy_tile = tuned_vy;
     if (mixed && z.vy_tile == 0) {
         // same choice as tile_plan_create: fewest wasted vy lanes
         const int nvy = z.L.max4 - z.L.min4 + 1;
@@ -974,7 +973,7 @@ JobSizes job_sizes(const smip_job_params& p)
     const int pad = mixed ? tile_plan_pad_needed(z.d, z.vy_tile) : 0;
     const int tail = mixed ? tile_plan_tail_needed(z.d, (int)p.nrows, z.vy_tile) : 0;
     z.hp = make_half_plane((int32_t)p.ncols, (int32_t)p.nrows, pad, tail);
-    z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (mixed ? 512 : 16);
+    z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (tuned ? tuned_batch : (mixed ? 512 : 16));
     const uint64_t X = p.ncols, Y = p.nrows, hx = X / 2 + 1, np = X * Y;
     uint64_t b = z.d.base_size * sizeof(double2) + hx * Y * sizeof(double);
     if (mixed)
@@ -992,22 +991,6 @@ JobSizes job_sizes(const smip_job_params& p)
 
 void record(smip_job* j, int kind, hipEvent_t a, hipEvent_t b) { j->events.push_back({ a, b, kind }); }
 
-// (u, v) tuples the accumulate loop of include/bispectrum.h:393-398 adds per frame
-uint64_t count_triples(const LoopBounds& L)
-{
-    uint64_t n = 0;
-    for (int32_t i = L.min1; i <= 0; ++i)
-        for (int32_t k = L.min3; k <= 0; ++k) {
-            if (i + k < L.min1 || i + k > L.max1)
-                continue;
-            for (int32_t j = L.min2; j <= L.max2; ++j) {
-                const int32_t lo = std::max(L.min4, L.min2 - j), hi = std::min(L.max4, L.max2 - j);
-                if (hi >= lo)
-                    n += (uint64_t)(hi - lo + 1);
-            }
-        }
-    return n;
-}
 
 hipEvent_t mark(smip_job* j)
 {
@@ -1101,6 +1084,18 @@ void grow_shifts(smip_job* j, uint64_t total)
 }
 
 } // namespace
+
+extern "C" SMIP_API int smip_job_resolve_params(const smip_job_params* p, smip_job_params* out)
+{
+    return guarded([&] {
+        require(p && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        const JobSizes z = job_sizes(*p);
+        smip_job_params q = *p;
+        q.frames_per_batch = z.batch;
+        q.vy_tile = p->precision == SMIP_PRECISION_MIXED ? z.vy_tile : p->vy_tile;
+        *out = q;
+    });
+}
 
 extern "C" SMIP_API int smip_job_footprint(const smip_job_params* p, uint64_t* bytes)
 {
@@ -1413,3 +1408,22 @@ extern "C" SMIP_API int smip_job_get_image(smip_job* j, double* out)
         SMIP_HIP(hipStreamSynchronize(j->stream));
     });
 }
+
+namespace smip_amd {
+// (u, v) tuples the accumulate loop of include/bispectrum.h:393-398 adds per frame
+uint64_t count_triples(const LoopBounds& L)
+{
+    uint64_t n = 0;
+    for (int32_t i = L.min1; i <= 0; ++i)
+        for (int32_t k = L.min3; k <= 0; ++k) {
+            if (i + k < L.min1 || i + k > L.max1)
+                continue;
+            for (int32_t j = L.min2; j <= L.max2; ++j) {
+                const int32_t lo = std::max(L.min4, L.min2 - j), hi = std::min(L.max4, L.max2 - j);
+                if (hi >= lo)
+                    n += (uint64_t)(hi - lo + 1);
+            }
+        }
+    return n;
+}
+} // namespace smip_amd

@@ -248,11 +248,15 @@ extern "C" SMIP_API int smip_server_submit(smip_server* s, const smip_job_params
     return guarded([&] {
         if (!s || !p || !id)
             fail(SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        // defaults resolved now, so the footprint admitted is the one created
+        smip_job_params q {};
+        if (smip_job_resolve_params(p, &q) != SMIP_OK)
+            fail(SMIP_ERR_INVALID_ARGUMENT, smip_last_error());
         uint64_t fp = 0;
-        if (smip_job_footprint(p, &fp) != SMIP_OK)
+        if (smip_job_footprint(&q, &fp) != SMIP_OK)
             fail(SMIP_ERR_INVALID_ARGUMENT, smip_last_error());
         auto j = std::make_unique<ServedJob>();
-        j->p = *p;
+        j->p = q;
         j->footprint = fp;
         j->frame_bytes = p->ncols * p->nrows * dtype_size(p->frame_dtype);
         j->t_submit = Clock::now();

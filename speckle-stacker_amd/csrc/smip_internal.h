@@ -245,4 +245,19 @@ void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, 
 int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,
     hipStream_t s);
 
+// batched-kernel autotuner cache (tune.hip); false when nothing is cached
+bool tune_lookup(uint64_t ncols, uint64_t nrows, uint64_t bdepth, int* vy_tile, int* batch);
+// (u, v) tuples the reference accumulate loop adds per frame
+uint64_t count_triples(const LoopBounds& L);
+
+} // namespace smip_amd
+
+// the device context behind smip_ctx handles
+struct smip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace smip_amd {
+inline void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
 } // namespace smip_amd

@@ -79,6 +79,11 @@ class ServerStats(C.Structure):
                 ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64), ("frames", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("vy_tile", C.c_int32), ("frames_per_batch", C.c_int32), ("frames_per_s", C.c_double),
+                ("tile_tflops", C.c_double), ("candidates", C.c_int32)]
+
+
 JOB_WAITING, JOB_ACTIVE, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 # name -> (restype, argtypes); every function in include/smip_mi355x.h
@@ -133,6 +138,9 @@ _SIGS = {
     "smip_job_get_shifts": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_get_phases": (C.c_int, [_P, _P, _P, _P]),
     "smip_job_get_image": (C.c_int, [_P, _P]),
+    "smip_job_resolve_params": (C.c_int, [_P, _P]),
+    "smip_autotune": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _P]),
+    "smip_autotune_forget": (C.c_int, []),
     "smip_server_create": (C.c_int, [_P, _P, _P]),
     "smip_server_destroy": (C.c_int, [_P]),
     "smip_server_submit": (C.c_int, [_P, _P, _P]),
@@ -418,6 +426,25 @@ def job_params(ncols, nrows, bdepth, reco_radius, frame_dtype, precision=PRECISI
                frames_per_batch=0, vy_tile=0) -> JobParams:
     return JobParams(ncols, nrows, bdepth, float(reco_radius), _NP_DTYPES[np.dtype(frame_dtype)], precision,
                      int(bool(saa)), frames_per_batch, vy_tile)
+
+
+def resolve_params(params: JobParams) -> JobParams:
+    """The parameters a job is created with (tuned or built-in defaults filled in)."""
+    out = JobParams()
+    _check(lib().smip_job_resolve_params(C.byref(params), C.byref(out)))
+    return out
+
+
+def autotune(ctx: Context, ncols: int, nrows: int, bdepth: int, max_bytes: int = 0) -> dict:
+    """Times the batched kernel's tile shapes and batch sizes; later jobs of this
+    geometry use the choice."""
+    r = TuneResult()
+    _check(lib().smip_autotune(ctx.h, ncols, nrows, bdepth, max_bytes, C.byref(r)))
+    return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+
+def autotune_forget() -> None:
+    _check(lib().smip_autotune_forget())
 
 
 def job_footprint(params: JobParams) -> int:

@@ -170,6 +170,21 @@ def test_engine_mixed(ctx, N, b, p, n, batch):
     assert _rel_l2(got.image, want.image) < 1e-5
 
 
+@pytest.mark.parametrize("precision", [S.PRECISION_FP64, S.PRECISION_MIXED])
+@pytest.mark.parametrize("b", [32, 40])
+def test_depth_at_and_beyond_half_frame(ctx, precision, b):
+    """b/2 >= N/2: the accumulate loops clip v to the frame (include/bispectrum.h:383-391)."""
+    N, p, n = 32, 12, 6
+    frames = _frames(n, N, seed=23)
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=precision)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.flags, want.flags)
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    assert _rel(got.bispectrum, want.bispectrum) < (1e-11 if precision == S.PRECISION_FP64 else 2e-5)
+    assert _rel_l2(got.image, want.image) < (1e-9 if precision == S.PRECISION_FP64 else 1e-5)
+
+
 @pytest.mark.parametrize("vy_tile", [9, 13, 17, 25, 33])
 def test_tile_shapes(ctx, vy_tile):
     """Every instantiated register-tile shape gives the same bispectrum."""
