@@ -199,6 +199,46 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
                    "binary/u8, triple/u16, extended/f32"}
 
 
+def precision_leg(ctx, n: int) -> dict:
+    """Error of the mixed path against the FP64 path on the 512x512 float32
+    extended-object stack (b = 64, p = 256), first n frames: bispectrum
+    magnitude and phase, phases and the image.  The FP64 GPU path is the
+    reference here (it matches the FP64 oracle to 1e-11 of max|B| on the
+    sizes the oracle finishes)."""
+    W = S.WORKLOADS["extended512"]
+    frames = S.synth_frames(S.synth_params("extended512"), 0, n, threads=min(16, os.cpu_count() or 1))
+    res = {}
+    for prec in (S.PRECISION_MIXED, S.PRECISION_FP64):
+        res[prec] = S.run_pipeline(ctx, frames, W["b"], W["p"], precision=prec, with_bispectrum=True)
+    m, d = res[S.PRECISION_MIXED], res[S.PRECISION_FP64]
+    bm, bd = m.bispectrum, d.bispectrum
+    amax = float(np.abs(bd).max())
+    mag_num = mag_den = 0.0
+    ph_w = ph_n = 0.0
+    ph_max = 0.0
+    step = 1 << 24
+    for k in range(0, bd.size, step):  # chunked: 4.5 GB per bispectrum
+        a, b = bm[k:k + step], bd[k:k + step]
+        aa, ab = np.abs(a), np.abs(b)
+        mag_num += float(np.sum((aa - ab) ** 2))
+        mag_den += float(np.sum(ab ** 2))
+        sig = ab > 1e-6 * amax
+        if sig.any():
+            dph = np.abs(np.angle(a[sig] * np.conj(b[sig])))
+            ph_w += float(np.sum(dph * ab[sig]))
+            ph_n += float(np.sum(ab[sig]))
+            ph_max = max(ph_max, float(dph.max()))
+    fl = m.flags.astype(bool) & d.flags.astype(bool)
+    return {"workload": f"extended512: first {n} of 10000 frames 512x512 float32, b={W['b']}, p={W['p']}",
+            "reference": "FP64 GPU path",
+            "bispectrum_magnitude_rel_l2": mag_num ** 0.5 / mag_den ** 0.5,
+            "bispectrum_phase_rad": {"weighted_mean": ph_w / ph_n, "max_where_|B|>1e-6max": ph_max},
+            "phase_max_abs_diff": float(np.abs(m.phases[fl] - d.phases[fl]).max()),
+            "flags_equal": bool(np.array_equal(m.flags, d.flags)),
+            "shifts_equal": bool(np.array_equal(m.shifts, d.shifts)),
+            "image_rel_l2": float(np.linalg.norm(m.image - d.image) / np.linalg.norm(d.image))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +255,8 @@ def main() -> int:
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson job arrivals per second")
+    ap.add_argument("--precision-frames", type=int, default=128,
+                    help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -325,6 +367,11 @@ def main() -> int:
             "shifts_equal": bool(np.array_equal(rm.shifts, rd.shifts)),
         }
         log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
+
+    if args.precision_frames > 0:
+        log(f"precision: mixed vs FP64 on {args.precision_frames} frames of 512x512, b=64, p=256")
+        out["precision_512"] = precision_leg(ctx, args.precision_frames)
+        log(f"precision: {out['precision_512']}")
 
     if args.serve_seconds > 0:
         log(f"serving: Poisson arrivals at {args.serve_rate} jobs/s for {args.serve_seconds} s")

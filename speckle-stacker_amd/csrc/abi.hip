@@ -678,7 +678,6 @@ extern "C" SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* 
 {
     return guarded([&] {
         require(dst && src, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
-        require(dst->dtype == src->dtype, SMIP_ERR_INVALID_ARGUMENT, "bispectrum element types differ");
         if (dst == src)
             return;
         ensure_device(dst->ctx);
@@ -686,12 +685,23 @@ extern "C" SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* 
             require(!dst->borrowed, SMIP_ERR_LOGIC, "Cannot resize a reference-backed Array_base.");
             if (dst->data)
                 SMIP_HIP(hipFree(dst->data));
-            dst->data = src->d.base_size ? dev_alloc<uint8_t>(src->d.base_size * src->esz()) : nullptr;
+            dst->data = nullptr;
+            dst->data = src->d.base_size ? dev_alloc<uint8_t>(src->d.base_size * dst->esz()) : nullptr;
         }
         dst->d = src->d;
         if (src->d.base_size) {
+            hipStream_t s = dst->ctx->stream;
             SMIP_HIP(hipStreamSynchronize(src->ctx->stream));
-            SMIP_HIP(hipMemcpy(dst->data, src->data, src->d.base_size * src->esz(), hipMemcpyDeviceToDevice));
+            const uint64_t n = src->d.base_size;
+            if (dst->dtype == src->dtype)
+                SMIP_HIP(hipMemcpyAsync(dst->data, src->data, n * src->esz(), hipMemcpyDeviceToDevice, s));
+            else if (dst->dtype == SMIP_DTYPE_C64) // elements converted, dst keeps its type
+                launch_convert_c64_to_c32(static_cast<const double2*>(src->data), static_cast<float2*>(dst->data), n,
+                    s);
+            else
+                launch_convert_c32_to_c64(static_cast<const float2*>(src->data), static_cast<double2*>(dst->data), n,
+                    s);
+            SMIP_HIP(hipStreamSynchronize(s));
         }
     });
 }

@@ -459,6 +459,21 @@ void launch_convert_c64_to_c32(const double2* in, float2* out, uint64_t n, hipSt
     SMIP_HIP(hipGetLastError());
 }
 
+__global__ void k_c32_to_c64(const float2* __restrict__ in, double2* __restrict__ out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = mk<double2>((double)in[i].x, (double)in[i].y);
+}
+
+void launch_convert_c32_to_c64(const float2* in, double2* out, uint64_t n, hipStream_t s)
+{
+    if (!n)
+        return;
+    hipLaunchKernelGGL(k_c32_to_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+    SMIP_HIP(hipGetLastError());
+}
+
 // get_element for a list of index tuples (device-side lookup with the reference's
 // canonicalisation and address arithmetic, aliasing included).
 __global__ void k_get_elements(Desc d, const void* __restrict__ B, int is_f32, const int32_t* __restrict__ idx,

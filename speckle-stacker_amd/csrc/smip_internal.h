@@ -219,6 +219,7 @@ uint64_t tile_plan_cells(const TilePlan* tp); // cells computed per frame
 void launch_scale_c64(double2* B, uint64_t n, double re, double im, int op, hipStream_t s);
 void launch_scale_c32(float2* B, uint64_t n, float re, float im, int op, hipStream_t s);
 void launch_convert_c64_to_c32(const double2* in, float2* out, uint64_t n, hipStream_t s);
+void launch_convert_c32_to_c64(const float2* in, double2* out, uint64_t n, hipStream_t s);
 void launch_get_elements(const Desc& d, const void* B, int is_f32, const int32_t* idx, int64_t n, double2* out,
     int32_t* status, hipStream_t s);
 

@@ -244,3 +244,19 @@ def test_triple256_sample(ctx):
     assert _rel(got.bispectrum, want.bispectrum) < 2e-5
     err = _rel_l2(got.image, want.image)
     assert err < 1e-5, err
+
+
+def test_assign_converts_element_type(ctx):
+    """operator= keeps the destination's element type (complex<double> <-> complex<float>)."""
+    rng = np.random.default_rng(3)
+    a = S.Bispectrum(ctx, (16, 16, 8, 8), np.complex128)
+    vals = (rng.standard_normal(a.download().shape) + 1j * rng.standard_normal(a.download().shape))
+    a.upload(vals)
+    f = S.Bispectrum(ctx, (8, 8, 4, 4), np.complex64)  # resized by the assignment
+    lib = S.lib()
+    S._check(lib.smip_bispec_assign(f.h, a.h))
+    got = f.download()
+    assert got.dtype == np.complex64 and np.array_equal(got, vals.astype(np.complex64))
+    back = S.Bispectrum(ctx, (16, 16, 8, 8), np.complex128)
+    S._check(lib.smip_bispec_assign(back.h, f.h))
+    assert np.array_equal(back.download(), vals.astype(np.complex64).astype(np.complex128))
