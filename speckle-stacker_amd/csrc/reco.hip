@@ -149,6 +149,33 @@ __device__ inline double wave_sum(double v)
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// count and complex sum of a wavefront at once: three independent DPP chains
+// interleaved, so their latencies overlap
+__device__ inline void wave_sum3(int& n, double& re, double& im)
+{
+#define SMIP_LEVEL(CTRL, ROWS)                                                                              \
+    {                                                                                                       \
+        const int tn = dpp_i<CTRL, ROWS>(n);                                                                \
+        const double tr = dpp_d<CTRL, ROWS>(re), ti = dpp_d<CTRL, ROWS>(im);                                \
+        n += tn;                                                                                            \
+        re += tr;                                                                                           \
+        im += ti;                                                                                           \
+    }
+    SMIP_LEVEL(0xB1, 0xf)
+    SMIP_LEVEL(0x4E, 0xf)
+    SMIP_LEVEL(0x141, 0xf)
+    SMIP_LEVEL(0x140, 0xf)
+    SMIP_LEVEL(0x142, 0xa)
+    SMIP_LEVEL(0x143, 0xc)
+#undef SMIP_LEVEL
+    n = __builtin_amdgcn_readlane(n, 63);
+    const long long br = __double_as_longlong(re), bi = __double_as_longlong(im);
+    re = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(br >> 32), 63) << 32)
+        | (unsigned int)__builtin_amdgcn_readlane((int)(br & 0xffffffff), 63));
+    im = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(bi >> 32), 63) << 32)
+        | (unsigned int)__builtin_amdgcn_readlane((int)(bi & 0xffffffff), 63));
+}
+
 __device__ inline int wave_sum(int v)
 {
     v += dpp_i<0xB1>(v);
@@ -168,13 +195,37 @@ __device__ inline void wave_lds_order()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// 1/sqrt(x) and 1/x from the hardware approximations (about 2^-23 relative)
+// and two Newton-Raphson steps, each squaring the relative error: 2^-46, then
+// rounding level.  A fraction of the latency of the IEEE sqrt and divide
+// sequences on the recursion's serial path; phases and consistencies carry a
+// tolerance, and the flags do not depend on them.
+__device__ inline double rsqrt_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-(x * y), y, 1.0); // 1 - x y^2
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+__device__ inline double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+
 __device__ inline void finalize_point(double sr, double si, int n, double2* phases, uint8_t* flags, double* cons,
     int32_t pw, double2* out_phase)
 {
     // mean = sum / n; consistency = |mean|; phase = mean / |mean| = sum / |sum|
-    const double r = sqrt(sr * sr + si * si);
-    const double ap = r / (double)n;
-    const double ia = 1.0 / r;
+    const double m2 = sr * sr + si * si;
+    const double ia = m2 > 0.0 ? rsqrt_nr(m2) : 0.0;
+    const double ap = m2 * ia * rcp_nr((double)n);
     const double2 ph = ap > kEps ? mk<double2>(sr * ia, si * ia) : mk<double2>(0.0, 0.0);
     flags[pw] = 1;
     cons[pw] = ap;
@@ -347,6 +398,7 @@ struct StepInfo {
     double2 S;   // pre-window sum
 };
 
+constexpr double kNotYet = 2.0;  // ppos marker: slot not flagged in this window
 constexpr int kChunk = 8;        // visits per staged chunk
 constexpr int kChunkRecs = 512;  // records per chunk buffer
 constexpr int kParts = sizeof(PendingRec) / sizeof(int4);
@@ -391,8 +443,9 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
     __shared__ PendingRec buf[kChunkRecs];
     __shared__ StepInfo info[kWindow];
     __shared__ int32_t ctotal[kWindow / kChunk];
+    // phase of each window slot once flagged in this window; x = kNotYet before
+    // (no unit or zero phase has x = 2), so one read gives flag and phase
     __shared__ double2 ppos[kWindow];
-    __shared__ uint8_t fpos[kWindow];
     const int lane = threadIdx.x;
     if (status[0])
         return;
@@ -412,10 +465,9 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
         si.off = 0;
         si.pad = 0;
         info[t] = si;
-        fpos[t] = 0;
     }
-    for (int t = nsteps + lane; t < kWindow; t += 64)
-        fpos[t] = 0;
+    for (int t = lane; t < kWindow; t += 64)
+        ppos[t] = mk<double2>(kNotYet, 0.0);
     __syncthreads();
     for (int c = lane; c < nchunks; c += 64) { // chunk-local record offsets
         int o = 0;
@@ -447,6 +499,12 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
         for (int j = q0; j < q1; ++j) {
             const StepInfo si = info[j];
             const PendingRec r0 = nr; // this visit's first 64 records, prefetched from LDS
+            // partner state of r0 first, then the next visit's prefetch behind it
+            double2 pu0 = mk<double2>(kNotYet, 0.0), pv0 = mk<double2>(kNotYet, 0.0);
+            if (staged && lane < si.cnt) {
+                pu0 = ppos[max(r0.ju, 0)];
+                pv0 = ppos[max(r0.jv, 0)];
+            }
             if (staged && j + 1 < q1) {
                 const StepInfo sn = info[j + 1];
                 nr = PendingRec {};
@@ -455,17 +513,14 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
             }
             if (si.n0 < 0)
                 continue;
-            if (si.rev && fpos[si.fp])
+            if (si.rev && ppos[si.fp].x != kNotYet)
                 continue; // flagged by an earlier visit of this window
             ++calls;
             double sr = 0.0, sim = 0.0;
             int ne = 0; // count + (error << 16)
-            auto take = [&](const PendingRec& r) {
-                const int ju = max(r.ju, 0), jv = max(r.jv, 0);
-                const uint8_t fu = fpos[ju], fv = fpos[jv];
-                const double2 pu = ppos[ju], pv = ppos[jv];
-                const bool uok = (r.f & kPreU) || (r.ju >= 0 && fu);
-                const bool vok = (r.f & kPreV) || (r.jv >= 0 && fv);
+            auto take = [&](const PendingRec& r, double2 pu, double2 pv) {
+                const bool uok = (r.f & kPreU) || (r.ju >= 0 && pu.x != kNotYet);
+                const bool vok = (r.f & kPreV) || (r.jv >= 0 && pv.x != kNotYet);
                 if (!(uok && vok))
                     return;
                 if (r.f & kErr) {
@@ -484,16 +539,17 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
                     ++ne;
                 }
             };
+            auto take_at = [&](const PendingRec& r) { take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)]); };
             if (staged) {
                 if (lane < si.cnt)
-                    take(r0);
+                    take(r0, pu0, pv0);
                 for (int t = 64 + lane; t < si.cnt; t += 64) // rare: more than 64 records
-                    take(buf[si.off + t]);
+                    take_at(buf[si.off + t]);
             } else {
                 for (int t = lane; t < si.cnt; t += 64)
-                    take(rec[(int64_t)j * cap + t]);
+                    take_at(rec[(int64_t)j * cap + t]);
             }
-            ne = wave_sum(ne);
+            wave_sum3(ne, sr, sim);
             if ((ne >> 16) || si.err) {
                 if (lane == 0)
                     status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
@@ -502,13 +558,10 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
             }
             const int tn = si.n0 + (ne & 0xffff);
             if (tn > 0) {
-                sr = wave_sum(sr);
-                sim = wave_sum(sim);
                 if (lane == 0) {
                     double2 ph;
                     finalize_point(si.S.x + sr, si.S.y + sim, tn, phases, flags, cons, si.pw, &ph);
                     ppos[si.fp] = ph;
-                    fpos[si.fp] = 1;
                 }
                 ++flagged;
                 wave_lds_order();
