@@ -203,7 +203,8 @@ typedef struct smip_job_stats {
     uint64_t cells_per_frame;   /* accumulator cells the tile kernel computes per frame */
     uint64_t triples_per_frame; /* (u,v) tuples the reference loop adds per frame */
     double tile_ms;             /* device time of the triple-product kernel launches */
-    uint64_t tile_launches;     /* number of those launches */
+    uint64_t tile_launches;     /* number of those launches (graph replays not included) */
+    uint64_t graph_launches;    /* per-batch HIP graph replays (full batches of pushed frames) */
 } smip_job_stats;
 
 /* Device bytes a job of these parameters holds (admission control). */

@@ -8,6 +8,7 @@
 // bispectrum runs in the HIP kernels of fft.hip, bispec.hip, reco.hip, post.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -145,6 +146,11 @@ struct smip_job {
     hipEvent_t pin_done[2] = { nullptr, nullptr };
     uint64_t pin_frames = 0; // frames per pinned slot
     int pin_cur = 0;
+    // per-batch launch sequences captured as HIP graphs (full batches from the
+    // upload buffer): accumulate, and shift-and-add into a fixed shift scratch
+    hipGraphExec_t g_acc = nullptr, g_saa = nullptr;
+    dptr<int32_t> bshift;
+    uint64_t graph_launches = 0;
     // outputs
     dptr<double2> phases, image;
     dptr<uint8_t> flags;
@@ -164,6 +170,10 @@ struct smip_job {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
+        if (g_acc)
+            (void)hipGraphExecDestroy(g_acc);
+        if (g_saa)
+            (void)hipGraphExecDestroy(g_saa);
         for (int k = 0; k < 2; ++k) {
             if (pin_done[k])
                 (void)hipEventDestroy(pin_done[k]);
@@ -992,7 +1002,7 @@ JobSizes job_sizes(const smip_job_params& p)
         b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
     if (p.saa)
         b += hx * Y * sizeof(double2) + (uint64_t)z.batch * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
-            + np * sizeof(double);
+            + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
     b += (uint64_t)z.batch * np * dtype_size(p.frame_dtype);             // host upload staging
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
     z.bytes = b;
@@ -1023,6 +1033,62 @@ void drain_events(smip_job* j)
     j->events.clear();
 }
 
+// FFT + |F|^2 + triple products of nf frames at fr (no timing events: also
+// captured into a graph)
+void enqueue_accumulate(smip_job* j, const uint8_t* fr, int64_t nf)
+{
+    const smip_job_params& p = j->p;
+    hipStream_t s = j->stream;
+    if (p.precision == SMIP_PRECISION_MIXED) {
+        launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H32.get(), j->work32.get(),
+            j->P_half.get(), s);
+        launch_accum_tile(j->tile, j->hp, j->H32.get(), nf, j->B64.get(), s);
+    } else {
+        launch_forward_f64(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H64.get(), j->work64.get(),
+            j->P_half.get(), s);
+        for (int64_t f = 0; f < nf; ++f) // one frame per launch: the plain drop-in
+            launch_accum_plain_f64(j->d, j->L, j->hp, j->H64.get() + f * j->hp.fs, j->B64.get(), s);
+    }
+}
+
+// shift-and-add of m frames at fm (none of them frame 0): shifts into sh
+void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh)
+{
+    const smip_job_params& p = j->p;
+    const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
+    hipStream_t s = j->stream;
+    launch_rfft_f64(j->plan, fm, p.frame_dtype, m, true, j->sspec.get(), j->swork.get(), s);
+    launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), m, j->corr.get(), j->swork.get(), s);
+    launch_argmax_shift(j->corr.get(), m, X, Y, sh, s);
+    launch_saa_add(fm, p.frame_dtype, m, X, Y, sh, j->sum.get(), s);
+}
+
+bool graphs_enabled()
+{
+    const char* e = std::getenv("SMIP_NO_GRAPH");
+    return !(e && e[0] == '1');
+}
+
+template <typename F>
+hipGraphExec_t capture_graph(hipStream_t s, F&& enqueue)
+{
+    SMIP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        enqueue();
+    } catch (...) {
+        if (hipStreamEndCapture(s, &g) == hipSuccess && g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    SMIP_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SMIP_HIP(e);
+    return ex;
+}
+
 void job_process(smip_job* j, const void* frames, uint64_t count)
 {
     const smip_job_params& p = j->p;
@@ -1033,8 +1099,16 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     for (uint64_t off = 0; off < count;) {
         const int64_t nf = (int64_t)std::min<uint64_t>((uint64_t)j->batch, count - off);
         const uint8_t* fr = static_cast<const uint8_t*>(frames) + off * fb;
+        // full batches from the upload buffer after frame 0 read and write the same
+        // buffers every time: one graph launch each instead of a dozen launches
+        const bool graph = graphs_enabled() && nf == j->batch && j->upload && fr == j->upload.get() && j->nframes > 0;
         hipEvent_t e0 = mark(j);
-        if (mixed) {
+        if (graph) {
+            if (!j->g_acc)
+                j->g_acc = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
+            SMIP_HIP(hipGraphLaunch(j->g_acc, s));
+            j->graph_launches++;
+        } else if (mixed) {
             launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H32.get(), j->work32.get(),
                 j->P_half.get(), s);
             hipEvent_t t0 = mark(j);
@@ -1043,32 +1117,32 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
             record(j, 4, t0, t1);
             j->tile_launches++;
         } else {
-            launch_forward_f64(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H64.get(), j->work64.get(),
-                j->P_half.get(), s);
-            for (int64_t f = 0; f < nf; ++f) // one frame per launch: the plain drop-in
-                launch_accum_plain_f64(j->d, j->L, j->hp, j->H64.get() + f * j->hp.fs, j->B64.get(), s);
+            enqueue_accumulate(j, fr, nf);
         }
         hipEvent_t e1 = mark(j);
         record(j, 0, e0, e1);
         if (p.saa) {
             hipEvent_t e2 = mark(j);
-            int64_t first = 0;
-            if (j->nframes == 0) {
-                // sumarray = Re(frame 0); the reference spectrum is frame 0's r2c
-                const int64_t np = (int64_t)X * Y;
-                hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, fr,
-                    p.frame_dtype, np, j->sum.get());
-                launch_rfft_f64(j->plan, fr, p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), s);
-                first = 1;
-            }
-            const int64_t m = nf - first;
-            if (m > 0) {
-                const uint8_t* fm = fr + first * fb;
-                launch_rfft_f64(j->plan, fm, p.frame_dtype, m, true, j->sspec.get(), j->swork.get(), s);
-                launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), m, j->corr.get(), j->swork.get(), s);
-                int32_t* sh = j->shifts.get() + 2 * (j->nframes + first);
-                launch_argmax_shift(j->corr.get(), m, X, Y, sh, s);
-                launch_saa_add(fm, p.frame_dtype, m, X, Y, sh, j->sum.get(), s);
+            if (graph) {
+                if (!j->g_saa)
+                    j->g_saa = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get()); });
+                SMIP_HIP(hipGraphLaunch(j->g_saa, s));
+                j->graph_launches++;
+                SMIP_HIP(hipMemcpyAsync(j->shifts.get() + 2 * j->nframes, j->bshift.get(), sizeof(int32_t) * 2 * nf,
+                    hipMemcpyDeviceToDevice, s));
+            } else {
+                int64_t first = 0;
+                if (j->nframes == 0) {
+                    // sumarray = Re(frame 0); the reference spectrum is frame 0's r2c
+                    const int64_t np = (int64_t)X * Y;
+                    hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, fr,
+                        p.frame_dtype, np, j->sum.get());
+                    launch_rfft_f64(j->plan, fr, p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), s);
+                    first = 1;
+                }
+                const int64_t m = nf - first;
+                if (m > 0)
+                    enqueue_saa(j, fr + first * fb, m, j->shifts.get() + 2 * (j->nframes + first));
             }
             hipEvent_t e3 = mark(j);
             record(j, 1, e2, e3);
@@ -1159,6 +1233,7 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
             j->corr = dmake<double>((size_t)j->batch * np);
             j->sum = dmake<double>(np);
             SMIP_HIP(hipMemsetAsync(j->sum.get(), 0, sizeof(double) * np, s));
+            j->bshift = dmake<int32_t>(2 * (size_t)j->batch);
         }
         j->phases = dmake<double2>(np);
         j->image = dmake<double2>(np);
@@ -1331,6 +1406,7 @@ extern "C" SMIP_API int smip_job_get_stats(smip_job* j, smip_job_stats* out)
         out->triples_per_frame = j->triples;
         out->tile_ms = j->ms[4];
         out->tile_launches = j->tile_launches;
+        out->graph_launches = j->graph_launches;
     });
 }
 

@@ -60,7 +60,8 @@ class JobStats(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("accumulate_ms", C.c_double), ("saa_ms", C.c_double),
                 ("reco_ms", C.c_double), ("synth_ms", C.c_double), ("footprint", C.c_uint64),
                 ("reco_calls", C.c_uint64), ("cells_per_frame", C.c_uint64),
-                ("triples_per_frame", C.c_uint64), ("tile_ms", C.c_double), ("tile_launches", C.c_uint64)]
+                ("triples_per_frame", C.c_uint64), ("tile_ms", C.c_double), ("tile_launches", C.c_uint64),
+                ("graph_launches", C.c_uint64)]
 
 
 class ServerParams(C.Structure):

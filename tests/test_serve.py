@@ -38,14 +38,14 @@ def _assert_identical(a, b):
 
 
 def test_push_is_independent_of_chunking(ctx):
-    frames = _frames("binary128", 64, 90)
+    frames = _frames("binary128", 64, 200)
     p = S.job_params(64, 64, 16, 24, frames.dtype, frames_per_batch=32)
     ref = S.Job(ctx, p)
-    ref.add_frames_host(frames)  # batches [0,32), [32,64), [64,90)
+    ref.add_frames(S.Frames(ctx, frames))  # device stack: eager launches, batches of 32 and a last of 8
     ref.finalize()
     want = _outputs(ref)
     ref.close()
-    for cuts in ([90], [1, 7, 33, 2, 47], [31, 1, 31, 1, 26]):
+    for cuts in ([200], [1, 7, 33, 2, 47, 110], [31, 1, 31, 1, 26, 40, 70]):
         job = S.Job(ctx, p)
         pos = 0
         for c in cuts:
@@ -54,6 +54,31 @@ def test_push_is_independent_of_chunking(ctx):
         job.finalize()
         _assert_identical(_outputs(job), want)
         job.close()
+
+
+@pytest.mark.parametrize("precision", [S.PRECISION_MIXED, S.PRECISION_FP64])
+def test_graph_replays_match_eager_launches(ctx, monkeypatch, precision):
+    """Full batches of pushed frames replay captured HIP graphs; every replay
+    (five here, with host work between them) must leave what the eager launches
+    leave."""
+    frames = _frames("triple256", 64, 6 * 24 + 5, seed_offset=7)
+    p = S.job_params(64, 64, 20, 30, frames.dtype, precision, frames_per_batch=24)
+    outs = {}
+    for mode in ("graph", "eager"):
+        if mode == "eager":
+            monkeypatch.setenv("SMIP_NO_GRAPH", "1")
+        job = S.Job(ctx, p)
+        for k in range(0, len(frames), 13):
+            job.push_frames_host(frames[k:k + 13])
+            job.stats()  # synchronises: eager host work between the replays
+        job.finalize()
+        st = job.stats()
+        assert st["frames"] == len(frames)
+        assert st["graph_launches"] == (10 if mode == "graph" else 0)  # 5 batches x (accumulate, shift-and-add)
+        outs[mode] = _outputs(job)
+        job.close()
+    monkeypatch.delenv("SMIP_NO_GRAPH")
+    _assert_identical(outs["graph"], outs["eager"])
 
 
 def test_push_mixing_rules(ctx):
