@@ -112,14 +112,7 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
     import heapq
 
     rng = np.random.default_rng(seed)
-    kinds = ["binary128", "triple256", "extended512"]
-    pools = {}
-    for N in (64, 128, 256, 512):
-        for kind in kinds:
-            prm = S.synth_params(kind)
-            prm.ncols = prm.nrows = N
-            prm.pupil_diameter = max(8.0, N / 4)
-            pools[(N, kind)] = S.synth_frames(prm, 0, 64, threads=min(16, os.cpu_count() or 1))
+    pools = frame_pools(ctx, host=True)
     # arrivals
     jobs = []
     t = 0.0
@@ -127,11 +120,9 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
         t += rng.exponential(1.0 / rate)
         if t >= seconds:
             break
-        N = int(rng.choice([64, 128, 256, 512]))
-        kind = kinds[int(rng.integers(0, 3))]
-        n = int(round(np.exp(rng.uniform(np.log(40), np.log(5000)))))
-        b = int(rng.integers(16, 65))
-        jobs.append(dict(t=t, N=N, kind=kind, n=n, b=b, p=2 * b, off=int(rng.integers(0, 64))))
+        q = draw_sequences(rng, 1)[0]
+        q["t"] = t
+        jobs.append(q)
     srv = S.Server(ctx)
     pending = []  # (time, seq, job index, first frame)
     for i, j in enumerate(jobs):
@@ -199,6 +190,106 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
                    "binary/u8, triple/u16, extended/f32"}
 
 
+def draw_sequences(rng, n: int) -> list:
+    """Sequences of the multi-sequence mix: side {64,128,256,512}, frame count
+    log-uniform in [40, 5000], b uniform in [16, 64] (at most 2N-1), p = 2b,
+    object and sample type per sequence."""
+    kinds = ["binary128", "triple256", "extended512"]
+    out = []
+    for _ in range(n):
+        N = int(rng.choice([64, 128, 256, 512]))
+        b = min(int(rng.integers(16, 65)), 2 * N - 1)
+        out.append(dict(N=N, kind=kinds[int(rng.integers(0, 3))],
+                        n=int(round(np.exp(rng.uniform(np.log(40), np.log(5000))))), b=b, p=2 * b,
+                        off=int(rng.integers(0, 64))))
+    return out
+
+
+def frame_pools(ctx, host: bool):
+    pools = {}
+    for N in (64, 128, 256, 512):
+        for kind in ("binary128", "triple256", "extended512"):
+            prm = S.synth_params(kind)
+            prm.ncols = prm.nrows = N
+            prm.pupil_diameter = max(8.0, N / 4)
+            f = S.synth_frames(prm, 0, 64, threads=min(16, os.cpu_count() or 1))
+            pools[(N, kind)] = f if host else S.Frames(ctx, np.concatenate([f, f]))  # 128: any 64-frame window
+    return pools
+
+
+def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
+    """nseq independent sequences processed together: `threads` host threads each
+    drive one sequence at a time through its own job and stream, frames from
+    device-resident pools (64 distinct frames per side and type, cycled)."""
+    rng = np.random.default_rng(0x5EC1E004)
+    seqs = draw_sequences(rng, nseq)
+    pools = frame_pools(ctx, host=False)
+    lat = [0.0] * nseq
+    errs = []
+    nxt = [0]
+    lock = threading.Lock()
+
+    def run(q, precision, limit=None):
+        dev = pools[(q["N"], q["kind"])]
+        job = S.Job(ctx, S.job_params(q["N"], q["N"], q["b"], q["p"], dev.dtype, precision))
+        n = q["n"] if limit is None else min(limit, q["n"])
+        pos = 0
+        while pos < n:
+            c = min(64, n - pos)
+            job.add_frames(dev, (q["off"] + pos) % 64, c)
+            pos += c
+        job.finalize()
+        st = job.stats()
+        job.close()
+        return st
+
+    def worker():
+        while True:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= nseq:
+                return
+            try:
+                run(seqs[i], S.PRECISION_MIXED)
+                lat[i] = (time.perf_counter() - t0) * 1e3
+            except Exception as e:  # reported, not hidden
+                errs.append(repr(e))
+
+    for q in seqs[:4]:  # warm-up: module load, plans
+        run(q, S.PRECISION_MIXED, limit=8)
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    if errs:
+        raise RuntimeError("; ".join(errs))
+    total = sum(q["n"] for q in seqs)
+    # plain serial FP64 path: one sequence at a time, first plain_sample frames of
+    # each, scaled to its full count (extrapolation from the timed sample)
+    est = 0.0
+    for q in seqs:
+        m = min(plain_sample, q["n"])
+        st = run(q, S.PRECISION_FP64, limit=m)
+        # per-frame stages scale with the frame count, recovery and synthesis do not
+        est += ((st["accumulate_ms"] + st["saa_ms"]) * q["n"] / m + st["reco_ms"] + st["synth_ms"]) * 1e-3
+    lat = np.array(lat)
+    return {"sequences": nseq, "frames": total, "host_threads": threads, "makespan_s": round(wall, 3),
+            "frames_per_s": round(total / wall, 1),
+            "sequence_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
+                                    "p99": round(float(np.percentile(lat, 99)), 1)},
+            "plain_fp64_serial_s_estimate": round(est, 2),
+            "speedup_vs_plain_fp64_serial": round(est / wall, 2),
+            "plain_estimate": f"device time of the plain FP64 path, one sequence at a time: accumulate and "
+                              f"shift-and-add timed on the first {plain_sample} frames of each sequence and scaled "
+                              "to its frame count, plus its phase recovery and synthesis",
+            "mix": "side {64,128,256,512}, frames log-uniform 40-5000, b U[16,64], p=2b, "
+                   "binary/u8, triple/u16, extended/f32; frames cycle through 64 per side and type"}
+
+
 def precision_leg(ctx, n: int) -> dict:
     """Error of the mixed path against the FP64 path on the 512x512 float32
     extended-object stack (b = 64, p = 256), first n frames: bispectrum
@@ -255,6 +346,7 @@ def main() -> int:
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson job arrivals per second")
+    ap.add_argument("--batch-seqs", type=int, default=64, help="sequences of the multi-sequence leg (0: skip)")
     ap.add_argument("--precision-frames", type=int, default=128,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
@@ -367,6 +459,11 @@ def main() -> int:
             "shifts_equal": bool(np.array_equal(rm.shifts, rd.shifts)),
         }
         log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
+
+    if args.batch_seqs > 0:
+        log(f"multi-sequence: {args.batch_seqs} sequences, 16 host threads")
+        out["multi_sequence"] = batch_leg(ctx, args.batch_seqs, 16, plain_sample=32)
+        log(f"multi-sequence: {out['multi_sequence']}")
 
     if args.precision_frames > 0:
         log(f"precision: mixed vs FP64 on {args.precision_frames} frames of 512x512, b=64, p=256")

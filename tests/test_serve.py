@@ -140,6 +140,14 @@ def test_result_is_the_same_alone_and_in_the_mix(ctx):
         prm = S.job_params(N, N, b, p, fr.dtype, frames_per_batch=16 + 8 * i)
         data.append((fr, prm))
     alone = [_run_alone(ctx, fr, prm) for fr, prm in data]
+    # each sequence against the FP64 oracle where it finishes in seconds (N <= 128)
+    for (fr, prm), a in zip(data, alone):
+        if prm.ncols > 128:
+            continue
+        want = O.pipeline(fr.astype(np.float64), prm.bdepth, prm.reco_radius)
+        assert np.array_equal(a["shifts"], want.shifts)
+        assert np.array_equal(a["flags"], want.flags)
+        assert np.linalg.norm(a["image"] - want.image) / np.linalg.norm(want.image) < 1e-5
 
     srv = S.Server(ctx, max_active=4, workers=4)
     ids = [None] * len(data)
