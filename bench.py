@@ -103,12 +103,15 @@ def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int):
     return lat, stats
 
 
-def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, camera_hz: float = 1000.0) -> dict:
+def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, camera_hz: float = 1000.0,
+              max_active: int = 64, drain_s: float = 90.0) -> dict:
     """The request loop under Poisson arrivals: jobs drawn as the multi-sequence
     mix (frame side from {64, 128, 256, 512}, frame count log-uniform in
     [40, 5000], b uniform in [16, 64], p = 2b, object and sample type per job),
     each streaming its frames in chunks of `chunk` at `camera_hz` per job.
-    Frames cycle through a pool of pre-generated frames per (side, type)."""
+    Frames cycle through a pool of pre-generated frames per (side, type).  The
+    run stops `drain_s` after the last arrival; jobs not finished by then are
+    reported as unfinished."""
     import heapq
 
     rng = np.random.default_rng(seed)
@@ -123,35 +126,44 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
         q = draw_sequences(rng, 1)[0]
         q["t"] = t
         jobs.append(q)
-    srv = S.Server(ctx)
+    srv = S.Server(ctx, max_active=max_active)
     pending = []  # (time, seq, job index, first frame)
     for i, j in enumerate(jobs):
         heapq.heappush(pending, (j["t"], i, i, 0))
     closed = []
-    done_lock = threading.Lock()
-    results = {}
+    results, failures = {}, {}
     stop = threading.Event()
 
     def reaper():
         k = 0
-        while not (stop.is_set() and k >= len(closed)):
+        while True:
             if k >= len(closed):
+                if stop.is_set():
+                    return
                 time.sleep(0.005)
                 continue
             i, jid = closed[k]
-            srv.wait(jid)
-            info = srv.info(jid)
-            srv.release(jid)
-            with done_lock:
-                results[i] = info
+            try:
+                while srv.wait(jid, 0.5) not in (S.JOB_DONE, S.JOB_FAILED):
+                    if stop.is_set():
+                        return
+                results[i] = srv.info(jid)
+            except Exception as e:  # reported, not hidden
+                failures[i] = repr(e)
+            try:
+                srv.release(jid)
+            except Exception:
+                pass
             k += 1
 
     th = threading.Thread(target=reaper)
     th.start()
     t0 = time.perf_counter()
+    last_log = t0
     frames_pushed = 0
     seq = len(jobs)
-    while pending:
+    deadline = t0 + seconds + drain_s
+    while pending and time.perf_counter() < deadline:
         when, _, i, first = heapq.heappop(pending)
         delay = t0 + when - time.perf_counter()
         if delay > 0:
@@ -161,8 +173,12 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
         if first == 0:
             j["id"] = srv.submit(S.job_params(j["N"], j["N"], j["b"], j["p"], pool.dtype))
         cnt = min(chunk, j["n"] - first)
-        idx = (np.arange(first, first + cnt) + j["off"]) % len(pool)
-        srv.push(j["id"], pool[idx])
+        pos = 0
+        while pos < cnt:  # contiguous pieces of the cycled pool, pushed without a host copy here
+            a = (j["off"] + first + pos) % len(pool)
+            c = min(cnt - pos, len(pool) - a)
+            srv.push(j["id"], pool[a:a + c])
+            pos += c
         frames_pushed += cnt
         if first + cnt >= j["n"]:
             srv.close_job(j["id"])
@@ -170,18 +186,33 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
         else:
             seq += 1
             heapq.heappush(pending, (when + cnt / camera_hz, seq, i, first + cnt))
+        now = time.perf_counter()
+        if now - last_log > 5.0:
+            st = srv.stats()
+            log(f"serving t={now - t0:.0f}s: submitted {st['submitted']}, done {st['completed']}, active "
+                f"{st['active']}, waiting {st['waiting']}, frames {st['frames']}/{frames_pushed}, "
+                f"lag {max(0.0, now - t0 - when):.2f}s")
+            last_log = now
+    while time.perf_counter() < deadline and len(results) + len(failures) < len(closed):
+        time.sleep(1.0)
+        st = srv.stats()
+        log(f"serving drain: done {st['completed']}/{len(jobs)}, active {st['active']}, waiting {st['waiting']}")
     stop.set()
     th.join()
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat = np.array([results[i]["latency_ms"] for i in range(len(jobs))])
-    tail = np.array([results[i]["tail_ms"] for i in range(len(jobs))])
-    queue = np.array([results[i]["queue_ms"] for i in range(len(jobs))])
+    done = sorted(results)
+    lat = np.array([results[i]["latency_ms"] for i in done]) if done else np.zeros(1)
+    tail = np.array([results[i]["tail_ms"] for i in done]) if done else np.zeros(1)
+    queue = np.array([results[i]["queue_ms"] for i in done]) if done else np.zeros(1)
     pct = lambda a, q: round(float(np.percentile(a, q)), 2)
-    return {"arrival": f"Poisson, {rate} jobs/s for {seconds} s", "jobs": len(jobs), "completed": st["completed"],
-            "failed": st["failed"], "frames": int(frames_pushed), "wall_s": round(wall, 2),
-            "frames_per_s": round(frames_pushed / wall, 1),
+    frames_done = int(sum(jobs[i]["n"] for i in done))
+    return {"arrival": f"Poisson, {rate} jobs/s for {seconds} s", "jobs": len(jobs), "completed": len(done),
+            "failed": len(failures), "unfinished": len(jobs) - len(done) - len(failures),
+            "failures": list(failures.values())[:3], "frames_of_completed_jobs": frames_done,
+            "frames_pushed": int(frames_pushed), "wall_s": round(wall, 2),
+            "frames_per_s": round(st["frames"] / wall, 1), "max_active": max_active,
             "latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
             "after_last_frame_ms": {"p50": pct(tail, 50), "p95": pct(tail, 95), "p99": pct(tail, 99)},
             "admission_wait_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)},
@@ -301,6 +332,7 @@ def precision_leg(ctx, n: int) -> dict:
     res = {}
     for prec in (S.PRECISION_MIXED, S.PRECISION_FP64):
         res[prec] = S.run_pipeline(ctx, frames, W["b"], W["p"], precision=prec, with_bispectrum=True)
+        log(f"precision: {'mixed' if prec == S.PRECISION_MIXED else 'FP64'} run done")
     m, d = res[S.PRECISION_MIXED], res[S.PRECISION_FP64]
     bm, bd = m.bispectrum, d.bispectrum
     amax = float(np.abs(bd).max())
@@ -320,7 +352,18 @@ def precision_leg(ctx, n: int) -> dict:
             ph_n += float(np.sum(ab[sig]))
             ph_max = max(ph_max, float(dph.max()))
     fl = m.flags.astype(bool) & d.flags.astype(bool)
+    log("precision: bispectrum errors done")
+    # conditioning: the FP64 path itself on the same frames moved by one float32 ulp
+    pert = np.nextafter(frames, np.float32(np.inf))
+    dp = S.run_pipeline(ctx, pert, W["b"], W["p"], precision=S.PRECISION_FP64, with_bispectrum=False)
+    cons = d.consistency[d.flags.astype(bool)]
     return {"workload": f"extended512: first {n} of 10000 frames 512x512 float32, b={W['b']}, p={W['p']}",
+            "fp64_vs_fp64_one_ulp_input": {
+                "image_rel_l2": float(np.linalg.norm(dp.image - d.image) / np.linalg.norm(d.image)),
+                "phase_max_abs_diff": float(np.abs(dp.phases[fl] - d.phases[fl]).max()),
+                "flags_equal": bool(np.array_equal(dp.flags, d.flags))},
+            "recovered_points": int(cons.size),
+            "consistency_below_1e-2": float(np.mean(cons < 1e-2)),
             "reference": "FP64 GPU path",
             "bispectrum_magnitude_rel_l2": mag_num ** 0.5 / mag_den ** 0.5,
             "bispectrum_phase_rad": {"weighted_mean": ph_w / ph_n, "max_where_|B|>1e-6max": ph_max},

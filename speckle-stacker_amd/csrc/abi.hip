@@ -945,6 +945,8 @@ extern "C" SMIP_API int smip_frames_upload(smip_frames* f, uint64_t first, uint6
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr int kSaaChunk = 64; // frames per shift-and-add pass (bounds its scratch)
+
 struct JobSizes {
     Desc d;
     LoopBounds L;
@@ -1001,7 +1003,8 @@ JobSizes job_sizes(const smip_job_params& p)
     else
         b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
     if (p.saa)
-        b += hx * Y * sizeof(double2) + (uint64_t)z.batch * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
+        b += hx * Y * sizeof(double2)
+            + (uint64_t)std::min(z.batch, kSaaChunk) * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
             + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
     b += (uint64_t)z.batch * np * dtype_size(p.frame_dtype);             // host upload staging
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
@@ -1051,16 +1054,22 @@ void enqueue_accumulate(smip_job* j, const uint8_t* fr, int64_t nf)
     }
 }
 
-// shift-and-add of m frames at fm (none of them frame 0): shifts into sh
+// shift-and-add of m frames at fm (none of them frame 0): shifts into sh; in
+// chunks of at most kSaaChunk frames, added in frame order
 void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh)
 {
     const smip_job_params& p = j->p;
     const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
+    const size_t fb = p.ncols * p.nrows * dtype_size(p.frame_dtype);
     hipStream_t s = j->stream;
-    launch_rfft_f64(j->plan, fm, p.frame_dtype, m, true, j->sspec.get(), j->swork.get(), s);
-    launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), m, j->corr.get(), j->swork.get(), s);
-    launch_argmax_shift(j->corr.get(), m, X, Y, sh, s);
-    launch_saa_add(fm, p.frame_dtype, m, X, Y, sh, j->sum.get(), s);
+    for (int64_t o = 0; o < m; o += kSaaChunk) {
+        const int64_t c = std::min<int64_t>(kSaaChunk, m - o);
+        const uint8_t* f = fm + o * fb;
+        launch_rfft_f64(j->plan, f, p.frame_dtype, c, true, j->sspec.get(), j->swork.get(), s);
+        launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), c, j->corr.get(), j->swork.get(), s);
+        launch_argmax_shift(j->corr.get(), c, X, Y, sh + 2 * o, s);
+        launch_saa_add(f, p.frame_dtype, c, X, Y, sh + 2 * o, j->sum.get(), s);
+    }
 }
 
 bool graphs_enabled()
@@ -1228,9 +1237,10 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         }
         if (p->saa) {
             j->ref_spec = dmake<double2>(hx * Y);
-            j->sspec = dmake<double2>((size_t)j->batch * hx * Y);
-            j->swork = dmake<double2>((size_t)j->batch * hx * Y);
-            j->corr = dmake<double>((size_t)j->batch * np);
+            const size_t sc = (size_t)std::min(j->batch, kSaaChunk);
+            j->sspec = dmake<double2>(sc * hx * Y);
+            j->swork = dmake<double2>(sc * hx * Y);
+            j->corr = dmake<double>(sc * np);
             j->sum = dmake<double>(np);
             SMIP_HIP(hipMemsetAsync(j->sum.get(), 0, sizeof(double) * np, s));
             j->bshift = dmake<int32_t>(2 * (size_t)j->batch);
