@@ -151,6 +151,11 @@ struct smip_job {
     hipGraphExec_t g_acc = nullptr, g_saa = nullptr;
     dptr<int32_t> bshift;
     uint64_t graph_launches = 0;
+    // device-stack path: the forward FFT of batch k+1 runs on fstream into the
+    // other half of a double-buffered spectrum while batch k's triple products run
+    hipStream_t fstream = nullptr;
+    dptr<float2> H32b;
+    hipEvent_t ev_fft[2] = { nullptr, nullptr }, ev_tile[2] = { nullptr, nullptr }, ev_fork = nullptr;
     // outputs
     dptr<double2> phases, image;
     dptr<uint8_t> flags;
@@ -170,6 +175,13 @@ struct smip_job {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
+        if (fstream) {
+            (void)hipStreamSynchronize(fstream);
+            (void)hipStreamDestroy(fstream);
+        }
+        for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork })
+            if (e)
+                (void)hipEventDestroy(e);
         if (g_acc)
             (void)hipGraphExecDestroy(g_acc);
         if (g_saa)
@@ -998,8 +1010,8 @@ JobSizes job_sizes(const smip_job_params& p)
     z.batch = p.frames_per_batch > 0 ? p.frames_per_batch : (tuned ? tuned_batch : (mixed ? 512 : 16));
     const uint64_t X = p.ncols, Y = p.nrows, hx = X / 2 + 1, np = X * Y;
     uint64_t b = z.d.base_size * sizeof(double2) + hx * Y * sizeof(double);
-    if (mixed)
-        b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(float2) + hx * Y * sizeof(float2));
+    if (mixed) // two spectra buffers (FFT / tile overlap on device stacks) + FFT work
+        b += (uint64_t)z.batch * (2 * (uint64_t)z.hp.fs * sizeof(float2) + hx * Y * sizeof(float2));
     else
         b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
     if (p.saa)
@@ -1098,6 +1110,20 @@ hipGraphExec_t capture_graph(hipStream_t s, F&& enqueue)
     return ex;
 }
 
+void enable_overlap(smip_job* j)
+{
+    if (j->fstream)
+        return;
+    j->H32b = dmake<float2>((size_t)j->batch * j->hp.fs);
+    SMIP_HIP(hipMemsetAsync(j->H32b.get(), 0, sizeof(float2) * j->batch * j->hp.fs, j->stream)); // padding rows
+    for (int k = 0; k < 2; ++k) {
+        SMIP_HIP(hipEventCreateWithFlags(&j->ev_fft[k], hipEventDisableTiming));
+        SMIP_HIP(hipEventCreateWithFlags(&j->ev_tile[k], hipEventDisableTiming));
+    }
+    SMIP_HIP(hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming));
+    SMIP_HIP(hipStreamCreateWithFlags(&j->fstream, hipStreamNonBlocking));
+}
+
 void job_process(smip_job* j, const void* frames, uint64_t count)
 {
     const smip_job_params& p = j->p;
@@ -1105,7 +1131,15 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
     hipStream_t s = j->stream;
     const bool mixed = p.precision == SMIP_PRECISION_MIXED;
-    for (uint64_t off = 0; off < count;) {
+    // frames from a device stack (not the reused upload buffer): FFT / tile overlap
+    const bool overlap = mixed && count > (uint64_t)j->batch && !(j->upload && frames == j->upload.get());
+    if (overlap) {
+        enable_overlap(j);
+        SMIP_HIP(hipEventRecord(j->ev_fork, s)); // fork: fstream follows everything before this call
+        SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_fork, 0));
+    }
+    int64_t k = 0; // batch index within this call
+    for (uint64_t off = 0; off < count; ++k) {
         const int64_t nf = (int64_t)std::min<uint64_t>((uint64_t)j->batch, count - off);
         const uint8_t* fr = static_cast<const uint8_t*>(frames) + off * fb;
         // full batches from the upload buffer after frame 0 read and write the same
@@ -1117,6 +1151,21 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
                 j->g_acc = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
             SMIP_HIP(hipGraphLaunch(j->g_acc, s));
             j->graph_launches++;
+        } else if (overlap) {
+            const int b = (int)(k & 1);
+            float2* H = b ? j->H32b.get() : j->H32.get();
+            if (k >= 2) // batch k-2's triple products have finished reading H
+                SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_tile[b], 0));
+            launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, H, j->work32.get(), j->P_half.get(),
+                j->fstream);
+            SMIP_HIP(hipEventRecord(j->ev_fft[b], j->fstream));
+            SMIP_HIP(hipStreamWaitEvent(s, j->ev_fft[b], 0));
+            hipEvent_t t0 = mark(j);
+            launch_accum_tile(j->tile, j->hp, H, nf, j->B64.get(), s);
+            hipEvent_t t1 = mark(j);
+            SMIP_HIP(hipEventRecord(j->ev_tile[b], s));
+            record(j, 4, t0, t1);
+            j->tile_launches++;
         } else if (mixed) {
             launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, j->H32.get(), j->work32.get(),
                 j->P_half.get(), s);
@@ -1158,6 +1207,10 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
         }
         j->nframes += (uint64_t)nf;
         off += (uint64_t)nf;
+    }
+    if (overlap) { // join: the power spectrum and spectra are complete for what follows on the main stream
+        SMIP_HIP(hipEventRecord(j->ev_fork, j->fstream));
+        SMIP_HIP(hipStreamWaitEvent(s, j->ev_fork, 0));
     }
 }
 
