@@ -140,14 +140,23 @@ def test_result_is_the_same_alone_and_in_the_mix(ctx):
         prm = S.job_params(N, N, b, p, fr.dtype, frames_per_batch=16 + 8 * i)
         data.append((fr, prm))
     alone = [_run_alone(ctx, fr, prm) for fr, prm in data]
-    # each sequence against the FP64 oracle where it finishes in seconds (N <= 128)
+    # each sequence against the FP64 oracle where it finishes in seconds (N <= 128):
+    # shifts and flags exact; the image within max(1e-5, 20 x the FP64 image's own
+    # change when every frame pixel moves by one float32 ulp), so that sequences
+    # whose phase recursion amplifies rounding (low-contrast extended objects,
+    # few frames) are held to their conditioning, not to a fixed number
     for (fr, prm), a in zip(data, alone):
         if prm.ncols > 128:
             continue
-        want = O.pipeline(fr.astype(np.float64), prm.bdepth, prm.reco_radius)
+        f64 = fr.astype(np.float64)
+        want = O.pipeline(f64, prm.bdepth, prm.reco_radius)
+        moved = O.pipeline(np.nextafter(fr.astype(np.float32), np.float32(np.inf)).astype(np.float64),
+                           prm.bdepth, prm.reco_radius)
+        norm = np.linalg.norm(want.image)
+        sens = np.linalg.norm(moved.image - want.image) / norm
         assert np.array_equal(a["shifts"], want.shifts)
         assert np.array_equal(a["flags"], want.flags)
-        assert np.linalg.norm(a["image"] - want.image) / np.linalg.norm(want.image) < 1e-5
+        assert np.linalg.norm(a["image"] - want.image) / norm <= max(1e-5, 20 * sens)
 
     srv = S.Server(ctx, max_active=4, workers=4)
     ids = [None] * len(data)
