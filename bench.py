@@ -390,9 +390,24 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson job arrivals per second")
     ap.add_argument("--batch-seqs", type=int, default=64, help="sequences of the multi-sequence leg (0: skip)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="hardware queues per process for HIP (GPU_MAX_HW_QUEUES, at most 32; 0: HIP's default)")
+    ap.add_argument("--only", default="", help="comma list of legs to run: latency,batch,precision,serve,cpu")
     ap.add_argument("--precision-frames", type=int, default=128,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            print(json.dumps({"error": "--hw-queues must be in [1, 32]"}))
+            return 2
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)  # read when HIP initialises (first context)
+    if args.only:
+        legs = set(args.only.split(","))
+        args.no_latency = "latency" not in legs
+        args.no_cpu = "cpu" not in legs
+        args.batch_seqs = args.batch_seqs if "batch" in legs else 0
+        args.precision_frames = args.precision_frames if "precision" in legs else 0
+        args.serve_seconds = args.serve_seconds if "serve" in legs else 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -469,7 +484,8 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {n} frames {N}x{N} {np.dtype(W['dtype']).name}, "
                                f"triple star, b={b}, p={pr}, mixed precision (FP32 batch, FP64 fold)",
                    "frames": n, "ncols": N, "nrows": N, "bdepth": b, "reco_radius": pr,
-                   "frames_per_batch": args.batch or 512, "triples_per_frame": triples},
+                   "frames_per_batch": args.batch or 512, "triples_per_frame": triples,
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "HIP default")},
         "plain_fp64_frames_per_s": round(fps64, 1),
         "speedup_vs_plain_fp64": round(fps / fps64, 2),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_FP32_PEAK_TFLOPS,

@@ -32,6 +32,84 @@ void set_last_error(const char* m)
 }
 void clear_last_error() { g_last_error[0] = 0; }
 const char* last_error_text() { return g_last_error; }
+
+std::recursive_mutex& capture_guard()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
+void* dev_malloc(size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(SMIP_ERR_NO_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return p;
+}
+
+void dev_free(void* p)
+{
+    if (!p)
+        return;
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    (void)hipFree(p);
+}
+
+void* host_malloc(size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(SMIP_ERR_NO_MEMORY, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return p;
+}
+
+void host_free(void* p)
+{
+    if (!p)
+        return;
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    (void)hipHostFree(p);
+}
+
+hipError_t mem_get_info(size_t* free_b, size_t* total_b)
+{
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    return hipMemGetInfo(free_b, total_b);
+}
+
+hipStream_t make_stream()
+{
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    hipStream_t s = nullptr;
+    SMIP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void destroy_stream(hipStream_t s)
+{
+    if (!s)
+        return;
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    (void)hipStreamDestroy(s);
+}
+
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    if (!bytes)
+        return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        SMIP_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+    }
+    SMIP_HIP(hipStreamSynchronize(s));
+}
 } // namespace smip_amd
 
 namespace {
@@ -47,20 +125,11 @@ T* dev_alloc(size_t n)
 {
     if (n == 0)
         return nullptr;
-    T* p = nullptr;
-    if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) {
-        (void)hipGetLastError();
-        fail(SMIP_ERR_NO_MEMORY, "device allocation of " + std::to_string(sizeof(T) * n) + " bytes failed");
-    }
-    return p;
+    return static_cast<T*>(dev_malloc(sizeof(T) * n));
 }
 
 struct DevFree {
-    void operator()(void* p) const
-    {
-        if (p)
-            (void)hipFree(p);
-    }
+    void operator()(void* p) const { dev_free(p); }
 };
 template <typename T>
 using dptr = std::unique_ptr<T, DevFree>;
@@ -85,7 +154,7 @@ struct smip_bispec {
     ~smip_bispec()
     {
         if (!borrowed && data)
-            (void)hipFree(data);
+            dev_free(data);
     }
 };
 
@@ -109,7 +178,7 @@ struct smip_frames {
     ~smip_frames()
     {
         if (data)
-            (void)hipFree(data);
+            dev_free(data);
     }
 };
 
@@ -175,10 +244,7 @@ struct smip_job {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
-        if (fstream) {
-            (void)hipStreamSynchronize(fstream);
-            (void)hipStreamDestroy(fstream);
-        }
+        destroy_stream(fstream);
         for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork })
             if (e)
                 (void)hipEventDestroy(e);
@@ -189,13 +255,11 @@ struct smip_job {
         for (int k = 0; k < 2; ++k) {
             if (pin_done[k])
                 (void)hipEventDestroy(pin_done[k]);
-            if (pin[k])
-                (void)hipHostFree(pin[k]);
+            host_free(pin[k]);
         }
         tile_plan_destroy(tile);
         fft_plan_destroy(plan);
-        if (stream)
-            (void)hipStreamDestroy(stream);
+        destroy_stream(stream);
     }
 };
 
@@ -367,7 +431,7 @@ extern "C" SMIP_API int smip_ctx_create(int32_t device, smip_ctx** out)
         auto c = std::make_unique<smip_ctx>();
         c->device = device;
         SMIP_HIP(hipSetDevice(device));
-        SMIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->stream = make_stream();
         *out = c.release();
     });
 }
@@ -377,7 +441,7 @@ extern "C" SMIP_API int smip_ctx_destroy(smip_ctx* ctx)
     return guarded([&] {
         if (!ctx)
             return;
-        (void)hipStreamDestroy(ctx->stream);
+        destroy_stream(ctx->stream);
         delete ctx;
     });
 }
@@ -397,7 +461,7 @@ extern "C" SMIP_API int smip_ctx_mem_info(smip_ctx* ctx, uint64_t* free_bytes, u
         require(ctx && free_bytes && total_bytes, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         ensure_device(ctx);
         size_t f = 0, t = 0;
-        SMIP_HIP(hipMemGetInfo(&f, &t));
+        SMIP_HIP(mem_get_info(&f, &t));
         *free_bytes = f;
         *total_bytes = t;
     });
@@ -562,10 +626,10 @@ extern "C" SMIP_API int smip_bispec_put_element(smip_bispec* b, const int32_t id
         ensure_device(b->ctx);
         if (b->dtype == SMIP_DTYPE_C128) {
             double2 v = make_double2(value[0], value[1]);
-            SMIP_HIP(hipMemcpy(static_cast<double2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice));
+            copy_sync(static_cast<double2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice, b->ctx->stream);
         } else {
             float2 v = make_float2((float)value[0], (float)value[1]);
-            SMIP_HIP(hipMemcpy(static_cast<float2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice));
+            copy_sync(static_cast<float2*>(b->data) + addr, &v, sizeof(v), hipMemcpyHostToDevice, b->ctx->stream);
         }
     });
 }
@@ -578,7 +642,7 @@ extern "C" SMIP_API int smip_bispec_download(const smip_bispec* b, void* host)
             return;
         ensure_device(b->ctx);
         SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
-        SMIP_HIP(hipMemcpy(host, b->data, b->d.base_size * b->esz(), hipMemcpyDeviceToHost));
+        copy_sync(host, b->data, b->d.base_size * b->esz(), hipMemcpyDeviceToHost, b->ctx->stream);
     });
 }
 
@@ -590,7 +654,7 @@ extern "C" SMIP_API int smip_bispec_upload(smip_bispec* b, const void* host)
             return;
         ensure_device(b->ctx);
         SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
-        SMIP_HIP(hipMemcpy(b->data, host, b->d.base_size * b->esz(), hipMemcpyHostToDevice));
+        copy_sync(b->data, host, b->d.base_size * b->esz(), hipMemcpyHostToDevice, b->ctx->stream);
     });
 }
 
@@ -602,7 +666,7 @@ extern "C" SMIP_API int smip_bispec_write_file(const smip_bispec* b, const char*
         if (b->d.base_size) {
             ensure_device(b->ctx);
             SMIP_HIP(hipStreamSynchronize(b->ctx->stream));
-            SMIP_HIP(hipMemcpy(buf.data(), b->data, buf.size(), hipMemcpyDeviceToHost));
+            copy_sync(buf.data(), b->data, buf.size(), hipMemcpyDeviceToHost, b->ctx->stream);
         }
         // header: size_t base_size, 4 x size_t constructor dims; then T[base_size]
         FILE* f = std::fopen(path, "wb");
@@ -640,7 +704,7 @@ extern "C" SMIP_API int smip_bispec_read_file(smip_bispec* b, const char* path)
         // Array_base::resize(size), then the descriptor of the stored dims
         if (size != b->d.base_size) {
             if (b->data)
-                SMIP_HIP(hipFree(b->data));
+                dev_free(b->data);
             b->data = size ? dev_alloc<uint8_t>(size * b->esz()) : nullptr;
         }
         const bool zero = !(hdr[1] || hdr[2] || hdr[3] || hdr[4]);
@@ -649,7 +713,7 @@ extern "C" SMIP_API int smip_bispec_read_file(smip_bispec* b, const char* path)
         if (got != size)
             fail(SMIP_ERR_RUNTIME, std::string("error reading data block from file ") + path);
         if (size)
-            SMIP_HIP(hipMemcpy(b->data, buf.data(), buf.size(), hipMemcpyHostToDevice));
+            copy_sync(b->data, buf.data(), buf.size(), hipMemcpyHostToDevice, b->ctx->stream);
     });
 }
 
@@ -706,7 +770,7 @@ extern "C" SMIP_API int smip_bispec_assign(smip_bispec* dst, const smip_bispec* 
         if (dst->d.base_size != src->d.base_size) {
             require(!dst->borrowed, SMIP_ERR_LOGIC, "Cannot resize a reference-backed Array_base.");
             if (dst->data)
-                SMIP_HIP(hipFree(dst->data));
+                dev_free(dst->data);
             dst->data = nullptr;
             dst->data = src->d.base_size ? dev_alloc<uint8_t>(src->d.base_size * dst->esz()) : nullptr;
         }
@@ -744,7 +808,7 @@ static void reconstruct_into(smip_ctx* ctx, hipStream_t s, const Desc& d, const 
     (void)ctx;
     const std::vector<int32_t> v = visit_list(reco_radius, (int64_t)xs, (int64_t)ys);
     std::unique_ptr<RecoWork, void (*)(RecoWork*)> w(
-        reco_work_create(d, (int32_t)xs, (int32_t)ys, v.data(), (int64_t)(v.size() / 2)), reco_work_destroy);
+        reco_work_create(d, (int32_t)xs, (int32_t)ys, v.data(), (int64_t)(v.size() / 2), s), reco_work_destroy);
     const int c = run_reconstruct(w.get(), B, is_f32, ph, fl, co, s);
     if (calls)
         *calls = (uint64_t)c;
@@ -814,7 +878,7 @@ extern "C" SMIP_API int smip_xcorr_create(smip_ctx* ctx, const double* ref, uint
         x->ctx = ctx;
         x->X = (int)ncols;
         x->Y = (int)nrows;
-        x->plan = fft_plan_create((int32_t)ncols, (int32_t)nrows);
+        x->plan = fft_plan_create((int32_t)ncols, (int32_t)nrows, ctx->stream);
         const uint64_t np = ncols * nrows, nh = (ncols / 2 + 1) * nrows;
         x->ref = dmake<double>(np);
         x->frame = dmake<double>(np);
@@ -859,7 +923,7 @@ extern "C" SMIP_API int smip_xcorr_get_correlation(smip_xcorr* x, double* out)
         if (x->readiness < 1)
             fail(SMIP_ERR_BAD_FUNCTION_CALL, "bad_function_call");
         ensure_device(x->ctx);
-        SMIP_HIP(hipMemcpy(out, x->corr.get(), sizeof(double) * x->X * x->Y, hipMemcpyDeviceToHost));
+        copy_sync(out, x->corr.get(), sizeof(double) * x->X * x->Y, hipMemcpyDeviceToHost, x->ctx->stream);
     });
 }
 
@@ -948,7 +1012,7 @@ extern "C" SMIP_API int smip_frames_upload(smip_frames* f, uint64_t first, uint6
         require(first + count <= f->count, SMIP_ERR_OUT_OF_RANGE, "frame range outside the stack");
         ensure_device(f->ctx);
         const size_t fb = f->X * f->Y * dtype_size(f->dtype);
-        SMIP_HIP(hipMemcpy(static_cast<uint8_t*>(f->data) + first * fb, host, count * fb, hipMemcpyHostToDevice));
+        copy_sync(static_cast<uint8_t*>(f->data) + first * fb, host, count * fb, hipMemcpyHostToDevice, f->ctx->stream);
     });
 }
 
@@ -1093,16 +1157,20 @@ bool graphs_enabled()
 template <typename F>
 hipGraphExec_t capture_graph(hipStream_t s, F&& enqueue)
 {
-    SMIP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
-    try {
-        enqueue();
-    } catch (...) {
-        if (hipStreamEndCapture(s, &g) == hipSuccess && g)
-            (void)hipGraphDestroy(g);
-        throw;
+    {
+        // no other thread of this library touches the legacy stream or allocates meanwhile
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        SMIP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue();
+        } catch (...) {
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        SMIP_HIP(hipStreamEndCapture(s, &g));
     }
-    SMIP_HIP(hipStreamEndCapture(s, &g));
     hipGraphExec_t ex = nullptr;
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1121,7 +1189,7 @@ void enable_overlap(smip_job* j)
         SMIP_HIP(hipEventCreateWithFlags(&j->ev_tile[k], hipEventDisableTiming));
     }
     SMIP_HIP(hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming));
-    SMIP_HIP(hipStreamCreateWithFlags(&j->fstream, hipStreamNonBlocking));
+    j->fstream = make_stream();
 }
 
 void job_process(smip_job* j, const void* frames, uint64_t count)
@@ -1258,7 +1326,7 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         ensure_device(ctx);
         const JobSizes z = job_sizes(*p);
         size_t fr = 0, tot = 0;
-        SMIP_HIP(hipMemGetInfo(&fr, &tot));
+        SMIP_HIP(mem_get_info(&fr, &tot));
         if (z.bytes > fr)
             fail(SMIP_ERR_NO_MEMORY, "job footprint " + std::to_string(z.bytes) + " B exceeds free device memory");
         auto j = std::make_unique<smip_job>();
@@ -1270,16 +1338,16 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         j->batch = z.batch;
         j->footprint = z.bytes;
         j->triples = count_triples(z.L);
-        SMIP_HIP(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+        j->stream = make_stream();
         hipStream_t s = j->stream;
         const uint64_t X = p->ncols, Y = p->nrows, hx = X / 2 + 1, np = X * Y;
-        j->plan = fft_plan_create((int32_t)X, (int32_t)Y);
+        j->plan = fft_plan_create((int32_t)X, (int32_t)Y, s);
         j->B64 = dmake<double2>(z.d.base_size);
         SMIP_HIP(hipMemsetAsync(j->B64.get(), 0, sizeof(double2) * z.d.base_size, s));
         j->P_half = dmake<double>(hx * Y);
         SMIP_HIP(hipMemsetAsync(j->P_half.get(), 0, sizeof(double) * hx * Y, s));
         if (p->precision == SMIP_PRECISION_MIXED) {
-            j->tile = tile_plan_create(z.d, z.L, z.hp, z.vy_tile);
+            j->tile = tile_plan_create(z.d, z.L, z.hp, z.vy_tile, s);
             j->H32 = dmake<float2>((size_t)j->batch * z.hp.fs);
             SMIP_HIP(hipMemsetAsync(j->H32.get(), 0, sizeof(float2) * j->batch * z.hp.fs, s)); // zero padding rows
             j->work32 = dmake<float2>((size_t)j->batch * hx * Y);
@@ -1379,7 +1447,7 @@ extern "C" SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frame
             // pinned slots of at most 8 MiB (at least one frame), never more than a batch
             j->pin_frames = std::max<uint64_t>(1, std::min<uint64_t>(batch, (8u << 20) / fb));
             for (int k = 0; k < 2; ++k) {
-                SMIP_HIP(hipHostMalloc((void**)&j->pin[k], j->pin_frames * fb, hipHostMallocDefault));
+                j->pin[k] = static_cast<uint8_t*>(host_malloc(j->pin_frames * fb));
                 SMIP_HIP(hipEventCreateWithFlags(&j->pin_done[k], hipEventDisableTiming));
                 SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
             }

@@ -278,7 +278,7 @@ static const int kVyTiles[] = { 9, 13, 17, 25, 33 };
 int tile_plan_vy_tile(const TilePlan* tp) { return tp->vy_tile; }
 uint64_t tile_plan_cells(const TilePlan* tp) { return tp->cells; }
 
-TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& hp, int vy_tile)
+TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& hp, int vy_tile, hipStream_t stream)
 {
     const int nvy = L.max4 - L.min4 + 1;
     if (vy_tile <= 0) { // fewest wasted vy lanes, then the larger tile
@@ -334,11 +334,14 @@ TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& 
     tp->nitems = (int64_t)perm.size();
     tp->cells = cells;
     tp->items = nullptr;
-    if (hipMalloc(&tp->items, sizeof(int4) * perm.size()) != hipSuccess) {
+    try {
+        tp->items = static_cast<int4*>(dev_malloc(sizeof(int4) * perm.size()));
+        copy_sync(tp->items, perm.data(), sizeof(int4) * perm.size(), hipMemcpyHostToDevice, stream);
+    } catch (...) {
+        dev_free(tp->items);
         delete tp;
-        fail(SMIP_ERR_NO_MEMORY, "tile plan allocation failed");
+        throw;
     }
-    SMIP_HIP(hipMemcpy(tp->items, perm.data(), sizeof(int4) * perm.size(), hipMemcpyHostToDevice));
     TileArgs& a = tp->args;
     a.X = X;
     a.Y = Y;
@@ -361,7 +364,7 @@ void tile_plan_destroy(TilePlan* tp)
 {
     if (!tp)
         return;
-    (void)hipFree(tp->items);
+    dev_free(tp->items);
     delete tp;
 }
 

@@ -37,7 +37,7 @@ static int ilog2(int n)
 }
 
 template <typename V>
-static V* upload_twiddles(int n)
+static V* upload_twiddles(int n, hipStream_t stream)
 {
     std::vector<V> h(n);
     const double tp = 6.283185307179586476925286766559;
@@ -49,13 +49,17 @@ static V* upload_twiddles(int n)
         h[k].x = (typename scalar_of<V>::type)c;
         h[k].y = (typename scalar_of<V>::type)(-s);
     }
-    V* d = nullptr;
-    SMIP_HIP(hipMalloc(&d, sizeof(V) * n));
-    SMIP_HIP(hipMemcpy(d, h.data(), sizeof(V) * n, hipMemcpyHostToDevice));
+    V* d = static_cast<V*>(dev_malloc(sizeof(V) * n));
+    try {
+        copy_sync(d, h.data(), sizeof(V) * n, hipMemcpyHostToDevice, stream);
+    } catch (...) {
+        dev_free(d);
+        throw;
+    }
     return d;
 }
 
-FftPlan* fft_plan_create(int32_t ncols, int32_t nrows)
+FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream)
 {
     auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
     if (!pow2(ncols) || !pow2(nrows) || ncols > 2048 || nrows > 2048)
@@ -66,10 +70,10 @@ FftPlan* fft_plan_create(int32_t ncols, int32_t nrows)
     p->lx = ilog2(ncols);
     p->ly = ilog2(nrows);
     try {
-        p->twx32 = upload_twiddles<float2>(ncols);
-        p->twy32 = upload_twiddles<float2>(nrows);
-        p->twx64 = upload_twiddles<double2>(ncols);
-        p->twy64 = upload_twiddles<double2>(nrows);
+        p->twx32 = upload_twiddles<float2>(ncols, stream);
+        p->twy32 = upload_twiddles<float2>(nrows, stream);
+        p->twx64 = upload_twiddles<double2>(ncols, stream);
+        p->twy64 = upload_twiddles<double2>(nrows, stream);
     } catch (...) {
         fft_plan_destroy(p);
         throw;
@@ -81,10 +85,10 @@ void fft_plan_destroy(FftPlan* p)
 {
     if (!p)
         return;
-    (void)hipFree(p->twx32);
-    (void)hipFree(p->twy32);
-    (void)hipFree(p->twx64);
-    (void)hipFree(p->twy64);
+    dev_free(p->twx32);
+    dev_free(p->twy32);
+    dev_free(p->twx64);
+    dev_free(p->twy64);
     delete p;
 }
 

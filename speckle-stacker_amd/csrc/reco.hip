@@ -684,12 +684,7 @@ __global__ __launch_bounds__(kSeqThreads) void k_reco_sequential(RecoArgs a, con
 template <typename T>
 static T* dalloc(size_t n)
 {
-    T* p = nullptr;
-    if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
-        (void)hipGetLastError();
-        fail(SMIP_ERR_NO_MEMORY, "phase-recovery work buffers do not fit in device memory");
-    }
-    return p;
+    return static_cast<T*>(dev_malloc(sizeof(T) * std::max<size_t>(n, 1)));
 }
 
 uint64_t reco_work_bytes(const Desc& d, int32_t xsize, int32_t ysize, int64_t nvisits)
@@ -701,7 +696,8 @@ uint64_t reco_work_bytes(const Desc& d, int32_t xsize, int32_t ysize, int64_t nv
         + (uint64_t)kWindow * cap * sizeof(PendingRec);
 }
 
-RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits)
+RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits,
+    hipStream_t stream)
 {
     auto* w = new RecoWork;
     RecoArgs& a = w->args;
@@ -746,8 +742,8 @@ RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const in
         w->rec = dalloc<PendingRec>((size_t)kWindow * w->cap);
         w->status = dalloc<int32_t>(4);
         if (nvisits) {
-            SMIP_HIP(hipMemcpy(w->visits, visits_host, sizeof(int2) * nvisits, hipMemcpyHostToDevice));
-            SMIP_HIP(hipMemcpy(w->first_pos, first.data(), sizeof(int32_t) * nvisits, hipMemcpyHostToDevice));
+            copy_sync(w->visits, visits_host, sizeof(int2) * nvisits, hipMemcpyHostToDevice, stream);
+            copy_sync(w->first_pos, first.data(), sizeof(int32_t) * nvisits, hipMemcpyHostToDevice, stream);
         }
     } catch (...) {
         reco_work_destroy(w);
@@ -762,8 +758,7 @@ void reco_work_destroy(RecoWork* w)
         return;
     for (void* p : { (void*)w->visits, (void*)w->first_pos, (void*)w->map, (void*)w->S, (void*)w->N, (void*)w->E,
              (void*)w->cnt, (void*)w->rec, (void*)w->status })
-        if (p)
-            (void)hipFree(p);
+        dev_free(p);
     delete w;
 }
 
@@ -834,7 +829,7 @@ void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, 
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s)
 {
     const int32_t v[2] = { wx, wy };
-    RecoWork* w = reco_work_create(d, xsize, ysize, v, 1);
+    RecoWork* w = reco_work_create(d, xsize, ysize, v, 1, s);
     try {
         const int npts = xsize * ysize;
         hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w->status);

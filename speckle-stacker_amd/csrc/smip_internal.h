@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -26,6 +27,22 @@ struct Error : std::runtime_error {
 };
 
 [[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+// HIP refuses calls that act on the legacy stream or the whole device
+// (synchronous copies; allocation and free may synchronise) while another thread
+// captures a stream into a graph.  Graph capture and every such call in this
+// library take this process-wide lock, and copies run on explicit non-blocking
+// streams (copy_sync), never on the legacy stream.
+std::recursive_mutex& capture_guard();
+void* dev_malloc(size_t bytes);                 // throws SMIP_ERR_NO_MEMORY
+void dev_free(void* p);                         // null allowed
+void* host_malloc(size_t bytes);                // pinned host memory
+void host_free(void* p);
+hipStream_t make_stream();                      // non-blocking, default priority
+hipError_t mem_get_info(size_t* free_b, size_t* total_b);
+void destroy_stream(hipStream_t s);
+// synchronous copy through stream s (which is synchronised)
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 
 // the calling thread's last error text (smip_last_error)
 void set_last_error(const char* msg);
@@ -180,7 +197,7 @@ enum class FrameType : int { u8 = SMIP_DTYPE_U8, u16 = SMIP_DTYPE_U16, f32 = SMI
 size_t dtype_size(int dtype);
 
 struct FftPlan; // twiddle tables for one frame shape
-FftPlan* fft_plan_create(int32_t ncols, int32_t nrows);
+FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream);
 void fft_plan_destroy(FftPlan*);
 
 // frames -> half-plane spectrum (float2 or double2), optional |x| of the input
@@ -207,7 +224,7 @@ void launch_accum_plain_f32(const Desc& d, const LoopBounds& lb, const HalfPlane
     float2* B, hipStream_t s);
 
 struct TilePlan; // work-item table of the batched accumulate
-TilePlan* tile_plan_create(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, int vy_tile);
+TilePlan* tile_plan_create(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, int vy_tile, hipStream_t stream);
 void tile_plan_destroy(TilePlan*);
 int tile_plan_vy_tile(const TilePlan*);
 int tile_plan_pad_needed(const Desc& d, int vy_tile);
@@ -238,8 +255,8 @@ void launch_power_expand(const double* power_half, int32_t ncols, int32_t nrows,
 
 // phase reconstruction (device); visits = in-range spiral visits (x,y) pairs
 struct RecoWork;
-RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host,
-    int64_t nvisits);
+RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits,
+    hipStream_t stream);
 void reco_work_destroy(RecoWork*);
 void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s);

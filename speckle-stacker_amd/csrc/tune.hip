@@ -118,20 +118,20 @@ extern "C" SMIP_API int smip_autotune(smip_ctx* ctx, uint64_t ncols, uint64_t nr
 
         ensure_device(ctx);
         hipStream_t s = ctx->stream;
-        float2* H = nullptr;
+        float2* H = static_cast<float2*>(dev_malloc((size_t)max_batch * per_frame));
         double2* B = nullptr;
-        SMIP_HIP(hipMalloc(&H, (size_t)max_batch * per_frame));
-        if (hipMalloc(&B, sizeof(double2) * d.base_size) != hipSuccess) {
-            (void)hipGetLastError();
-            (void)hipFree(H);
-            fail(SMIP_ERR_NO_MEMORY, "autotune: bispectrum allocation failed");
+        try {
+            B = static_cast<double2*>(dev_malloc(sizeof(double2) * d.base_size));
+        } catch (...) {
+            dev_free(H);
+            throw;
         }
         std::vector<TilePlan*> plans;
         auto cleanup = [&] {
             for (auto* t : plans)
                 tile_plan_destroy(t);
-            (void)hipFree(H);
-            (void)hipFree(B);
+            dev_free(H);
+            dev_free(B);
         };
         try {
             const int64_t n = (int64_t)max_batch * hp.fs;
@@ -160,7 +160,7 @@ extern "C" SMIP_API int smip_autotune(smip_ctx* ctx, uint64_t ncols, uint64_t nr
             double best_ms = 1e30;
             TilePlan* best_plan = nullptr;
             for (int t : kTiles) {
-                TilePlan* tp = tile_plan_create(d, L, hp, t);
+                TilePlan* tp = tile_plan_create(d, L, hp, t, s);
                 plans.push_back(tp);
                 const double ms = time_launch(tp, probe);
                 if (ms < best_ms) {

@@ -242,3 +242,42 @@ def test_server_errors(ctx):
     with pytest.raises(S.SmipError):
         srv.info(jid)
     srv.close()
+
+
+def test_captures_concurrent_with_allocation_and_copies(ctx):
+    """Graph captures (second full batch of every job) overlap with other threads'
+    job creation, plan uploads, frame uploads and frees; every job must succeed
+    and equal its result computed alone.  HIP refuses legacy-stream copies while a
+    stream captures, so the library never issues them and serialises capture
+    against allocation."""
+    frames = _frames("binary128", 64, 40, seed_offset=11)
+    p = S.job_params(64, 64, 16, 20, frames.dtype, frames_per_batch=8)
+    ref = S.Job(ctx, p)
+    ref.add_frames(S.Frames(ctx, frames))
+    ref.finalize()
+    want = ref.image()
+    ref.close()
+    errors, bad = [], []
+
+    def worker(t):
+        try:
+            for k in range(6):
+                job = S.Job(ctx, p)
+                dev = S.Frames(ctx, frames)  # an upload and an allocation between captures
+                for c in range(0, 40, 5):
+                    job.push_frames_host(frames[c:c + 5])
+                job.finalize()
+                if not np.array_equal(job.image(), want):
+                    bad.append((t, k))
+                dev.close()
+                job.close()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors[:3]
+    assert not bad, bad
