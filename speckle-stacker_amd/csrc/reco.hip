@@ -219,14 +219,23 @@ __device__ inline double rcp_nr(double x)
     return y;
 }
 
-__device__ inline void finalize_point(double sr, double si, int n, double2* phases, uint8_t* flags, double* cons,
-    int32_t pw, double2* out_phase)
+// phase sum/|sum| and consistency |sum|/n of a point (include/phasereco.h:107-121)
+__device__ inline double2 point_phase(double sr, double si, int n, double* consistency)
 {
     // mean = sum / n; consistency = |mean|; phase = mean / |mean| = sum / |sum|
     const double m2 = sr * sr + si * si;
     const double ia = m2 > 0.0 ? rsqrt_nr(m2) : 0.0;
     const double ap = m2 * ia * rcp_nr((double)n);
-    const double2 ph = ap > kEps ? mk<double2>(sr * ia, si * ia) : mk<double2>(0.0, 0.0);
+    *consistency = ap;
+    return ap > kEps ? mk<double2>(sr * ia, si * ia) : mk<double2>(0.0, 0.0);
+}
+
+__device__ inline void finalize_point(double sr, double si, int n, double2* phases, uint8_t* flags, double* cons,
+    int32_t pw, double2* out_phase)
+{
+    // mean = sum / n; consistency = |mean|; phase = mean / |mean| = sum / |sum|
+    double ap;
+    const double2 ph = point_phase(sr, si, n, &ap);
     flags[pw] = 1;
     cons[pw] = ap;
     phases[pw] = ph;
@@ -440,12 +449,15 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
     double* __restrict__ cons, int32_t* __restrict__ status)
 {
-    __shared__ PendingRec buf[kChunkRecs];
+    __shared__ PendingRec buf[2][kChunkRecs]; // double-buffered record staging
     __shared__ StepInfo info[kWindow];
     __shared__ int32_t ctotal[kWindow / kChunk];
-    // phase of each window slot once flagged in this window; x = kNotYet before
-    // (no unit or zero phase has x = 2), so one read gives flag and phase
+    // phase and consistency of each window slot once flagged in this window;
+    // ppos x = kNotYet before (no unit or zero phase has x = 2).  The window's
+    // results go to global memory once, at the end: within the window every
+    // reader of them goes through these arrays.
     __shared__ double2 ppos[kWindow];
+    __shared__ double pcons[kWindow];
     const int lane = threadIdx.x;
     if (status[0])
         return;
@@ -480,21 +492,25 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
     __syncthreads();
 
     const int4* src4 = reinterpret_cast<const int4*>(rec);
+    if (nchunks > 0 && ctotal[0] <= kChunkRecs)
+        fix_stage_chunk(reinterpret_cast<int4*>(buf[0]), src4, info, 0, nsteps, ctotal[0] * kParts, cap, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
     int calls = 0, flagged = 0;
     bool aborted = false;
     for (int c = 0; c < nchunks && !aborted; ++c) {
+        const PendingRec* cb = buf[c & 1];
         const bool staged = ctotal[c] <= kChunkRecs;
-        if (staged) { // one burst of direct global->LDS loads, one wait
-            fix_stage_chunk(reinterpret_cast<int4*>(buf), src4, info, c, nsteps, ctotal[c] * kParts, cap, lane);
-            __builtin_amdgcn_s_waitcnt(0);
-        }
-        __syncthreads();
+        // the next chunk's records travel while this one is processed
+        if (c + 1 < nchunks && ctotal[c + 1] <= kChunkRecs)
+            fix_stage_chunk(reinterpret_cast<int4*>(buf[(c + 1) & 1]), src4, info, c + 1, nsteps,
+                ctotal[c + 1] * kParts, cap, lane);
         const int q0 = c * kChunk, q1 = min(nsteps, q0 + kChunk);
         PendingRec nr {};
         if (staged) {
             const StepInfo i0 = info[q0];
             if (lane < i0.cnt)
-                nr = buf[i0.off + lane];
+                nr = cb[i0.off + lane];
         }
         for (int j = q0; j < q1; ++j) {
             const StepInfo si = info[j];
@@ -509,7 +525,7 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
                 const StepInfo sn = info[j + 1];
                 nr = PendingRec {};
                 if (lane < sn.cnt)
-                    nr = buf[sn.off + lane];
+                    nr = cb[sn.off + lane];
             }
             if (si.n0 < 0)
                 continue;
@@ -544,7 +560,7 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
                 if (lane < si.cnt)
                     take(r0, pu0, pv0);
                 for (int t = 64 + lane; t < si.cnt; t += 64) // rare: more than 64 records
-                    take_at(buf[si.off + t]);
+                    take_at(cb[si.off + t]);
             } else {
                 for (int t = lane; t < si.cnt; t += 64)
                     take_at(rec[(int64_t)j * cap + t]);
@@ -559,15 +575,26 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
             const int tn = si.n0 + (ne & 0xffff);
             if (tn > 0) {
                 if (lane == 0) {
-                    double2 ph;
-                    finalize_point(si.S.x + sr, si.S.y + sim, tn, phases, flags, cons, si.pw, &ph);
-                    ppos[si.fp] = ph;
+                    double ap;
+                    ppos[si.fp] = point_phase(si.S.x + sr, si.S.y + sim, tn, &ap);
+                    pcons[si.fp] = ap;
                 }
                 ++flagged;
                 wave_lds_order();
             }
         }
+        __builtin_amdgcn_s_waitcnt(0); // the next chunk's records have landed
         __syncthreads();
+    }
+    // the window's results, once
+    for (int t = lane; t < nsteps; t += 64) {
+        const double2 ph = ppos[t];
+        if (ph.x != kNotYet) {
+            const int32_t pw = info[t].pw;
+            flags[pw] = 1;
+            cons[pw] = pcons[t];
+            phases[pw] = ph;
+        }
     }
     if (lane == 0) {
         atomicAdd(&status[1], calls);
