@@ -498,7 +498,6 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
     __syncthreads();
     int calls = 0, flagged = 0;
     bool aborted = false;
-    const double2 kNot = mk<double2>(kNotYet, 0.0);
     for (int c = 0; c < nchunks && !aborted; ++c) {
         const PendingRec* cb = buf[c & 1];
         const bool staged = ctotal[c] <= kChunkRecs;
@@ -506,47 +505,32 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
         if (c + 1 < nchunks && ctotal[c + 1] <= kChunkRecs)
             fix_stage_chunk(reinterpret_cast<int4*>(buf[(c + 1) & 1]), src4, info, c + 1, nsteps,
                 ctotal[c + 1] * kParts, cap, lane);
-        const int q0 = c * kChunk, nk = min(nsteps, q0 + kChunk) - q0;
-        // this chunk's visit scalars in lanes 0..nk-1, read back with readlane
-        StepInfo mi {};
-        if (lane < nk)
-            mi = info[q0 + lane];
-        // Visit k's pending records, their partners' window phases and the
-        // revisit flag, loaded one visit ahead: they travel while the previous
-        // visit computes.  The previous visit's own result is patched in from
-        // registers once known (every earlier write precedes the loads).
-        struct Pre {
-            PendingRec r;
-            double2 pu, pv;
-            bool rev_done;
-        };
-        auto prefetch = [&](int k) {
-            Pre p;
-            p.r = PendingRec {};
-            p.pu = kNot;
-            p.pv = kNot;
-            const int cnt = __builtin_amdgcn_readlane(mi.cnt, k), off = __builtin_amdgcn_readlane(mi.off, k);
-            if (lane < cnt) {
-                p.r = staged ? cb[off + lane] : rec[(int64_t)(q0 + k) * cap + lane];
-                p.pu = ppos[max(p.r.ju, 0)];
-                p.pv = ppos[max(p.r.jv, 0)];
+        const int q0 = c * kChunk, q1 = min(nsteps, q0 + kChunk);
+        PendingRec nr {};
+        if (staged) {
+            const StepInfo i0 = info[q0];
+            if (lane < i0.cnt)
+                nr = cb[i0.off + lane];
+        }
+        for (int j = q0; j < q1; ++j) {
+            const StepInfo si = info[j];
+            const PendingRec r0 = nr; // this visit's first 64 records, prefetched from LDS
+            // partner state of r0 first, then the next visit's prefetch behind it
+            double2 pu0 = mk<double2>(kNotYet, 0.0), pv0 = mk<double2>(kNotYet, 0.0);
+            if (staged && lane < si.cnt) {
+                pu0 = ppos[max(r0.ju, 0)];
+                pv0 = ppos[max(r0.jv, 0)];
             }
-            p.rev_done = ppos[__builtin_amdgcn_readlane(mi.fp, k)].x != kNotYet;
-            return p;
-        };
-        Pre cur = prefetch(0);
-        for (int k = 0; k < nk; ++k) {
-            Pre nxt {};
-            if (k + 1 < nk)
-                nxt = prefetch(k + 1);
-            const int n0 = __builtin_amdgcn_readlane(mi.n0, k);
-            const int fp = __builtin_amdgcn_readlane(mi.fp, k);
-            const int cnt = __builtin_amdgcn_readlane(mi.cnt, k);
-            const int off = __builtin_amdgcn_readlane(mi.off, k);
-            if (n0 < 0 || (__builtin_amdgcn_readlane(mi.rev, k) && cur.rev_done)) {
-                cur = nxt; // nothing to do, or flagged by an earlier visit of this window
+            if (staged && j + 1 < q1) {
+                const StepInfo sn = info[j + 1];
+                nr = PendingRec {};
+                if (lane < sn.cnt)
+                    nr = cb[sn.off + lane];
+            }
+            if (si.n0 < 0)
                 continue;
-            }
+            if (si.rev && ppos[si.fp].x != kNotYet)
+                continue; // flagged by an earlier visit of this window
             ++calls;
             double sr = 0.0, sim = 0.0;
             int ne = 0; // count + (error << 16)
@@ -571,43 +555,33 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
                     ++ne;
                 }
             };
-            if (lane < cnt)
-                take(cur.r, cur.pu, cur.pv);
-            for (int t = 64 + lane; t < cnt; t += 64) { // rare: more than 64 records
-                const PendingRec r = staged ? cb[off + t] : rec[(int64_t)(q0 + k) * cap + t];
-                take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)]);
+            auto take_at = [&](const PendingRec& r) { take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)]); };
+            if (staged) {
+                if (lane < si.cnt)
+                    take(r0, pu0, pv0);
+                for (int t = 64 + lane; t < si.cnt; t += 64) // rare: more than 64 records
+                    take_at(cb[si.off + t]);
+            } else {
+                for (int t = lane; t < si.cnt; t += 64)
+                    take_at(rec[(int64_t)j * cap + t]);
             }
             wave_sum3(ne, sr, sim);
-            if ((ne >> 16) || __builtin_amdgcn_readlane(mi.err, k)) {
+            if ((ne >> 16) || si.err) {
                 if (lane == 0)
                     status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
                 aborted = true;
                 break;
             }
-            const int tn = n0 + (ne & 0xffff);
+            const int tn = si.n0 + (ne & 0xffff);
             if (tn > 0) {
-                const long long bx = __double_as_longlong(mi.S.x), by = __double_as_longlong(mi.S.y);
-                const double s0 = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(bx >> 32), k) << 32)
-                    | (unsigned int)__builtin_amdgcn_readlane((int)(bx & 0xffffffff), k));
-                const double s1 = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(by >> 32), k) << 32)
-                    | (unsigned int)__builtin_amdgcn_readlane((int)(by & 0xffffffff), k));
-                double ap;
-                const double2 ph = point_phase(s0 + sr, s1 + sim, tn, &ap); // uniform over the lanes
                 if (lane == 0) {
-                    ppos[fp] = ph;
-                    pcons[fp] = ap;
+                    double ap;
+                    ppos[si.fp] = point_phase(si.S.x + sr, si.S.y + sim, tn, &ap);
+                    pcons[si.fp] = ap;
                 }
                 ++flagged;
-                // the next visit's loads were issued before this write
-                if (nxt.r.ju == fp)
-                    nxt.pu = ph;
-                if (nxt.r.jv == fp)
-                    nxt.pv = ph;
-                if (k + 1 < nk && __builtin_amdgcn_readlane(mi.fp, k + 1) == fp)
-                    nxt.rev_done = true;
                 wave_lds_order();
             }
-            cur = nxt;
         }
         __builtin_amdgcn_s_waitcnt(0); // the next chunk's records have landed
         __syncthreads();
