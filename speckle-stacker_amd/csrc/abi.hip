@@ -11,6 +11,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <tuple>
+#include <mutex>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -225,6 +228,9 @@ struct smip_job {
     hipStream_t fstream = nullptr;
     dptr<float2> H32b;
     hipEvent_t ev_fft[2] = { nullptr, nullptr }, ev_tile[2] = { nullptr, nullptr }, ev_fork = nullptr;
+    // ... and the shift-and-add runs beside both on its own stream
+    hipStream_t sstream = nullptr;
+    hipEvent_t ev_sjoin = nullptr;
     // outputs
     dptr<double2> phases, image;
     dptr<uint8_t> flags;
@@ -245,7 +251,8 @@ struct smip_job {
             (void)hipEventDestroy(e.b);
         }
         destroy_stream(fstream);
-        for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork })
+        destroy_stream(sstream);
+        for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork, ev_sjoin })
             if (e)
                 (void)hipEventDestroy(e);
         if (g_acc)
@@ -310,7 +317,7 @@ void next_reco_index(double& r, double& phi, int32_t& i, int32_t& j)
 
 // Visits of reconstruct_phases' loop (include/phasereco.h:56-68) that fall in
 // the PhaseMap range; the others are no-ops.
-std::vector<int32_t> visit_list(double reco_radius, int64_t X, int64_t Y)
+std::vector<int32_t> make_visit_list(double reco_radius, int64_t X, int64_t Y)
 {
     const int32_t mn0 = -(int32_t)(X >> 1), mn1 = -(int32_t)(Y >> 1);
     const int32_t mx0 = -(int32_t)X / 2 + (int32_t)X - 1, mx1 = -(int32_t)Y / 2 + (int32_t)Y - 1;
@@ -324,6 +331,27 @@ std::vector<int32_t> visit_list(double reco_radius, int64_t X, int64_t Y)
             v.push_back(j);
         }
     }
+    return v;
+}
+
+// The list depends only on (radius, extents) and costs tens of thousands of
+// libm cos/sin steps: kept per process (at most 64 geometries).
+std::shared_ptr<const std::vector<int32_t>> visit_list(double reco_radius, int64_t X, int64_t Y)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<double, int64_t, int64_t>, std::shared_ptr<const std::vector<int32_t>>> cache;
+    const auto key = std::make_tuple(reco_radius, X, Y);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end())
+            return it->second;
+    }
+    auto v = std::make_shared<const std::vector<int32_t>>(make_visit_list(reco_radius, X, Y));
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 64)
+        cache.clear();
+    cache[key] = v;
     return v;
 }
 
@@ -806,9 +834,9 @@ static void reconstruct_into(smip_ctx* ctx, hipStream_t s, const Desc& d, const 
 {
     check_reco_geometry(d, xs, ys);
     (void)ctx;
-    const std::vector<int32_t> v = visit_list(reco_radius, (int64_t)xs, (int64_t)ys);
+    const auto v = visit_list(reco_radius, (int64_t)xs, (int64_t)ys);
     std::unique_ptr<RecoWork, void (*)(RecoWork*)> w(
-        reco_work_create(d, (int32_t)xs, (int32_t)ys, v.data(), (int64_t)(v.size() / 2), s), reco_work_destroy);
+        reco_work_create(d, (int32_t)xs, (int32_t)ys, v->data(), (int64_t)(v->size() / 2), s), reco_work_destroy);
     const int c = run_reconstruct(w.get(), B, is_f32, ph, fl, co, s);
     if (calls)
         *calls = (uint64_t)c;
@@ -1056,17 +1084,7 @@ JobSizes job_sizes(const smip_job_params& p)
     if (mixed && z.vy_tile == 0 && tuned)
         z.vy_tile = tuned_vy;
     if (mixed && z.vy_tile == 0) {
-        // same choice as tile_plan_create: fewest wasted vy lanes
-        const int nvy = z.L.max4 - z.L.min4 + 1;
-        int best = 9, bw = 1 << 30;
-        for (int c : { 9, 13, 17, 25, 33 }) {
-            const int w = ((nvy + c - 1) / c) * c - nvy;
-            if (w < bw || (w == bw && c > best)) {
-                best = c;
-                bw = w;
-            }
-        }
-        z.vy_tile = best;
+        z.vy_tile = tile_default_vy(z.L.max4 - z.L.min4 + 1); // as tile_plan_create
     }
     const int pad = mixed ? tile_plan_pad_needed(z.d, z.vy_tile) : 0;
     const int tail = mixed ? tile_plan_tail_needed(z.d, (int)p.nrows, z.vy_tile) : 0;
@@ -1091,13 +1109,16 @@ JobSizes job_sizes(const smip_job_params& p)
 void record(smip_job* j, int kind, hipEvent_t a, hipEvent_t b) { j->events.push_back({ a, b, kind }); }
 
 
-hipEvent_t mark(smip_job* j)
+hipEvent_t mark_on(smip_job* j, hipStream_t s)
 {
+    (void)j;
     hipEvent_t e;
     SMIP_HIP(hipEventCreate(&e));
-    SMIP_HIP(hipEventRecord(e, j->stream));
+    SMIP_HIP(hipEventRecord(e, s));
     return e;
 }
+
+hipEvent_t mark(smip_job* j) { return mark_on(j, j->stream); }
 
 void drain_events(smip_job* j)
 {
@@ -1132,12 +1153,11 @@ void enqueue_accumulate(smip_job* j, const uint8_t* fr, int64_t nf)
 
 // shift-and-add of m frames at fm (none of them frame 0): shifts into sh; in
 // chunks of at most kSaaChunk frames, added in frame order
-void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh)
+void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh, hipStream_t s)
 {
     const smip_job_params& p = j->p;
     const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
     const size_t fb = p.ncols * p.nrows * dtype_size(p.frame_dtype);
-    hipStream_t s = j->stream;
     for (int64_t o = 0; o < m; o += kSaaChunk) {
         const int64_t c = std::min<int64_t>(kSaaChunk, m - o);
         const uint8_t* f = fm + o * fb;
@@ -1190,6 +1210,10 @@ void enable_overlap(smip_job* j)
     }
     SMIP_HIP(hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming));
     j->fstream = make_stream();
+    if (j->p.saa) {
+        SMIP_HIP(hipEventCreateWithFlags(&j->ev_sjoin, hipEventDisableTiming));
+        j->sstream = make_stream();
+    }
 }
 
 void job_process(smip_job* j, const void* frames, uint64_t count)
@@ -1201,10 +1225,15 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     const bool mixed = p.precision == SMIP_PRECISION_MIXED;
     // frames from a device stack (not the reused upload buffer): FFT / tile overlap
     const bool overlap = mixed && count > (uint64_t)j->batch && !(j->upload && frames == j->upload.get());
+    hipStream_t ss = s; // stream of the shift-and-add
     if (overlap) {
         enable_overlap(j);
-        SMIP_HIP(hipEventRecord(j->ev_fork, s)); // fork: fstream follows everything before this call
+        SMIP_HIP(hipEventRecord(j->ev_fork, s)); // fork: fstream (and sstream) follow everything before this call
         SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_fork, 0));
+        if (p.saa) {
+            ss = j->sstream;
+            SMIP_HIP(hipStreamWaitEvent(ss, j->ev_fork, 0));
+        }
     }
     int64_t k = 0; // batch index within this call
     for (uint64_t off = 0; off < count; ++k) {
@@ -1248,10 +1277,10 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
         hipEvent_t e1 = mark(j);
         record(j, 0, e0, e1);
         if (p.saa) {
-            hipEvent_t e2 = mark(j);
+            hipEvent_t e2 = mark_on(j, ss);
             if (graph) {
                 if (!j->g_saa)
-                    j->g_saa = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get()); });
+                    j->g_saa = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get(), s); });
                 SMIP_HIP(hipGraphLaunch(j->g_saa, s));
                 j->graph_launches++;
                 SMIP_HIP(hipMemcpyAsync(j->shifts.get() + 2 * j->nframes, j->bshift.get(), sizeof(int32_t) * 2 * nf,
@@ -1261,24 +1290,28 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
                 if (j->nframes == 0) {
                     // sumarray = Re(frame 0); the reference spectrum is frame 0's r2c
                     const int64_t np = (int64_t)X * Y;
-                    hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, fr,
+                    hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ss, fr,
                         p.frame_dtype, np, j->sum.get());
-                    launch_rfft_f64(j->plan, fr, p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), s);
+                    launch_rfft_f64(j->plan, fr, p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), ss);
                     first = 1;
                 }
                 const int64_t m = nf - first;
                 if (m > 0)
-                    enqueue_saa(j, fr + first * fb, m, j->shifts.get() + 2 * (j->nframes + first));
+                    enqueue_saa(j, fr + first * fb, m, j->shifts.get() + 2 * (j->nframes + first), ss);
             }
-            hipEvent_t e3 = mark(j);
+            hipEvent_t e3 = mark_on(j, ss);
             record(j, 1, e2, e3);
         }
         j->nframes += (uint64_t)nf;
         off += (uint64_t)nf;
     }
-    if (overlap) { // join: the power spectrum and spectra are complete for what follows on the main stream
+    if (overlap) { // join: the power spectrum, spectra and sum image are complete for what follows on the main stream
         SMIP_HIP(hipEventRecord(j->ev_fork, j->fstream));
         SMIP_HIP(hipStreamWaitEvent(s, j->ev_fork, 0));
+        if (p.saa) {
+            SMIP_HIP(hipEventRecord(j->ev_sjoin, ss));
+            SMIP_HIP(hipStreamWaitEvent(s, j->ev_sjoin, 0));
+        }
     }
 }
 

@@ -275,23 +275,33 @@ __global__ __launch_bounds__(64) void k_accum_tile(const int4* __restrict__ item
 
 static const int kVyTiles[] = { 9, 13, 17, 25, 33 };
 
+// Default vy tile for nvy valid vy values: least modelled time, chunks x tile
+// width x a per-width factor.  The factor is 1 except for 33, whose register
+// footprint halves occupancy: measured 1.65x the time per cell of 17 and 25 at
+// 256x256, b = 48.  Per-item overhead is about 0.2 cell and is ignored.  Ties go
+// to the wider tile.
+int tile_default_vy(int nvy)
+{
+    int best = kVyTiles[0];
+    double best_cost = 1e30;
+    for (int c : kVyTiles) {
+        const double cost = (double)((nvy + c - 1) / c) * c * (c == 33 ? 1.65 : 1.0);
+        if (cost < best_cost || (cost == best_cost && c > best)) {
+            best = c;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 int tile_plan_vy_tile(const TilePlan* tp) { return tp->vy_tile; }
 uint64_t tile_plan_cells(const TilePlan* tp) { return tp->cells; }
 
 TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& hp, int vy_tile, hipStream_t stream)
 {
     const int nvy = L.max4 - L.min4 + 1;
-    if (vy_tile <= 0) { // fewest wasted vy lanes, then the larger tile
-        int best = kVyTiles[0], best_waste = 1 << 30;
-        for (int c : kVyTiles) {
-            const int chunks = (nvy + c - 1) / c, waste = chunks * c - nvy;
-            if (waste < best_waste || (waste == best_waste && c > best)) {
-                best = c;
-                best_waste = waste;
-            }
-        }
-        vy_tile = best;
-    }
+    if (vy_tile <= 0)
+        vy_tile = tile_default_vy(nvy);
     bool ok = false;
     for (int c : kVyTiles)
         ok |= (c == vy_tile);
