@@ -497,6 +497,17 @@ def main() -> int:
 
     # ---- complete jobs: latency and accuracy ---------------------------------
     if not args.no_latency:
+        log("running 5 complete jobs one after another (accumulate + SAA + phases + synthesis)")
+        single = []
+        for _ in range(5):
+            t = time.perf_counter()
+            lat1, st1 = run_latency_jobs(ctx, dev, W, 1, args.batch, args.vy_tile)
+            single.append((time.perf_counter() - t) * 1e3)
+        out["single_job_latency_ms"] = {"p50": round(float(np.percentile(single, 50)), 2),
+                                        "p99": round(float(np.percentile(single, 99)), 2), "jobs": 5,
+                                        "frames_per_job": n, "stages_ms": {k: round(st1[0][k], 2) for k in
+                                        ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}}
+        log(f"single job latency {out['single_job_latency_ms']}")
         log(f"running {args.jobs} concurrent complete jobs (accumulate + SAA + phases + synthesis)")
         lat, stats = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile)
         lat.sort()
