@@ -229,6 +229,15 @@ SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frames, uint64_t
 SMIP_API int smip_job_resolve_params(const smip_job_params* p, smip_job_params* out);
 /* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376). */
 SMIP_API int smip_job_finalize(smip_job* j);
+/* smip_job_finalize of n jobs at once (no counterpart in the reference): their
+ * phase recoveries run as one group, one launch per stage and window index
+ * for all of them, so the recursions of different jobs proceed side by side.
+ * Each job's results equal those of smip_job_finalize.  Jobs must share a
+ * device; none may be finalized or appear twice.  status (may be NULL)
+ * receives each job's code; a job whose recovery fails (SMIP_ERR_OUT_OF_RANGE,
+ * where get_element throws) is left unfinalized and the others complete; the
+ * return value is the first failure. */
+SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n, int32_t* status);
 SMIP_API int smip_job_synchronize(smip_job* j);
 SMIP_API int smip_job_get_stats(smip_job* j, smip_job_stats* out);
 /* The job's bispectrum as a Bispectrum object (borrowed; valid until the job is
@@ -300,6 +309,8 @@ typedef struct smip_server_stats {
     uint64_t waiting, active;       /* now */
     uint64_t bytes_in_use, budget;  /* admitted footprints / budget */
     uint64_t frames;                /* frames processed */
+    uint64_t finalize_groups;       /* smip_job_finalize_many calls: jobs ready together finalise together */
+    uint64_t finalized_in_groups;   /* jobs those calls held */
 } smip_server_stats;
 
 SMIP_API int smip_server_create(smip_ctx* ctx, const smip_server_params* p, smip_server** out);

@@ -1543,6 +1543,133 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
     });
 }
 
+extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n, int32_t* status)
+{
+    int first_err = SMIP_OK;
+    std::string first_msg;
+    const int rc = guarded([&] {
+        require(jobs != nullptr || n == 0, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (n == 0)
+            return;
+        require(n <= 4096, SMIP_ERR_INVALID_ARGUMENT, "too many jobs in one group");
+        for (uint64_t k = 0; k < n; ++k) {
+            require(jobs[k] != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null job");
+            require(!jobs[k]->finalized, SMIP_ERR_LOGIC, "job already finalized");
+            require(jobs[k]->ctx->device == jobs[0]->ctx->device, SMIP_ERR_INVALID_ARGUMENT,
+                "jobs of one group must share a device");
+            for (uint64_t m = 0; m < k; ++m)
+                require(jobs[m] != jobs[k], SMIP_ERR_INVALID_ARGUMENT, "a job appears twice in the group");
+        }
+        for (uint64_t k = 0; k < n; ++k) { // geometry first: nothing is changed for an invalid group
+            check_reco_geometry(jobs[k]->d, jobs[k]->p.ncols, jobs[k]->p.nrows);
+            require(jobs[k]->nframes + jobs[k]->staged > 0, SMIP_ERR_RUNTIME,
+                "Array_base<T>::operator/=(T) : division by zero");
+        }
+        auto note = [&](uint64_t k, int code, const char* msg) {
+            if (status)
+                status[k] = code;
+            if (code != SMIP_OK && first_err == SMIP_OK) {
+                first_err = code;
+                first_msg = msg;
+            }
+        };
+        const char* seq = std::getenv("SMIP_RECO_SEQUENTIAL");
+        if (seq && seq[0] == '1') { // the one-visit-at-a-time cross-check runs job by job
+            for (uint64_t k = 0; k < n; ++k) {
+                const int r = smip_job_finalize(jobs[k]);
+                note(k, r, last_error_text());
+            }
+            return;
+        }
+        ensure_device(jobs[0]->ctx);
+        hipStream_t cs = jobs[0]->stream; // the group's stream
+        std::vector<hipEvent_t> ev;
+        std::vector<RecoWork*> work;
+        auto cleanup = [&] {
+            for (hipEvent_t e : ev)
+                (void)hipEventDestroy(e);
+            for (RecoWork* w : work)
+                reco_work_destroy(w);
+        };
+        try {
+            auto event = [&](bool timing) {
+                hipEvent_t e;
+                SMIP_HIP(timing ? hipEventCreate(&e) : hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                ev.push_back(e);
+                return e;
+            };
+            for (uint64_t k = 0; k < n; ++k) {
+                smip_job* j = jobs[k];
+                if (j->staged > 0) { // the last, partial batch of pushed frames
+                    job_process(j, j->upload.get(), j->staged);
+                    j->staged = 0;
+                }
+                // bispectrum /= complex(n, 0)
+                launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, j->stream);
+                hipEvent_t e = event(false);
+                SMIP_HIP(hipEventRecord(e, j->stream));
+                SMIP_HIP(hipStreamWaitEvent(cs, e, 0));
+            }
+            std::vector<const void*> B(n);
+            std::vector<int> f32(n, 0);
+            std::vector<double2*> ph(n);
+            std::vector<uint8_t*> fl(n);
+            std::vector<double*> co(n);
+            std::vector<int32_t> calls(n, 0), st(n, 0);
+            for (uint64_t k = 0; k < n; ++k) {
+                smip_job* j = jobs[k];
+                const auto v = visit_list(j->p.reco_radius, (int64_t)j->p.ncols, (int64_t)j->p.nrows);
+                work.push_back(reco_work_create(j->d, (int32_t)j->p.ncols, (int32_t)j->p.nrows, v->data(),
+                    (int64_t)(v->size() / 2), cs));
+                B[k] = j->B64.get();
+                ph[k] = j->phases.get();
+                fl[k] = j->flags.get();
+                co[k] = j->cons.get();
+            }
+            hipEvent_t t0 = event(true), t1 = event(true);
+            SMIP_HIP(hipEventRecord(t0, cs));
+            run_reconstruct_many(work.data(), (int)n, B.data(), f32.data(), ph.data(), fl.data(), co.data(),
+                calls.data(), st.data(), cs);
+            SMIP_HIP(hipEventRecord(t1, cs));
+            SMIP_HIP(hipEventSynchronize(t1));
+            float reco_ms = 0.f;
+            SMIP_HIP(hipEventElapsedTime(&reco_ms, t0, t1));
+            for (uint64_t k = 0; k < n; ++k) {
+                smip_job* j = jobs[k];
+                const smip_job_params& p = j->p;
+                j->reco_calls = (uint64_t)calls[k];
+                j->ms[2] += reco_ms; // the group's recovery time, shared
+                note(k, st[k], "reconstruct_phases: bispectrum element address out of bounds");
+                if (st[k] != SMIP_OK)
+                    continue; // left unfinalized, as smip_job_finalize leaves a job that throws
+                SMIP_HIP(hipStreamWaitEvent(j->stream, t1, 0));
+                hipEvent_t e2 = mark(j);
+                const double norm = (double)(j->nframes * p.ncols * p.nrows);
+                launch_synth_prepare(j->P_half.get(), j->phases.get(), (int32_t)p.ncols, (int32_t)p.nrows,
+                    p.reco_radius, norm, j->image.get(), j->stream);
+                launch_c2c_inverse_f64(j->plan, j->image.get(), nullptr, j->stream);
+                launch_abs_max_normalize(j->image.get(), (int64_t)(p.ncols * p.nrows), j->scratch.get(), j->stream);
+                hipEvent_t e3 = mark(j);
+                record(j, 3, e2, e3);
+            }
+            for (uint64_t k = 0; k < n; ++k) {
+                SMIP_HIP(hipStreamSynchronize(jobs[k]->stream));
+                if (st[k] == SMIP_OK)
+                    jobs[k]->finalized = true;
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+    if (rc != SMIP_OK)
+        return rc;
+    if (first_err != SMIP_OK)
+        set_last_error(first_msg.c_str());
+    return first_err;
+}
+
 extern "C" SMIP_API int smip_job_synchronize(smip_job* j)
 {
     return guarded([&] {

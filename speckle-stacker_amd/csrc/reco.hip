@@ -246,10 +246,11 @@ __device__ inline void finalize_point(double sr, double si, int n, double2* phas
 // ---------------------------------------------------------------------------
 // windowed schedule
 // ---------------------------------------------------------------------------
-__global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __restrict__ first_pos, RecoArgs a,
-    int64_t s, int64_t e, int64_t win, int64_t* __restrict__ map)
+__device__ __forceinline__ void reco_map_body(int bx, const int2* __restrict__ visits,
+    const int32_t* __restrict__ first_pos, const RecoArgs& a, int64_t s, int64_t e, int64_t win,
+    int64_t* __restrict__ map)
 {
-    const int64_t k = s + blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = s + (int64_t)bx * blockDim.x + threadIdx.x;
     if (k >= e)
         return;
     if (first_pos[k] == (int32_t)(k - s)) {
@@ -258,7 +259,7 @@ __global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __res
     }
 }
 
-__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits, int64_t s,
+__device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const int2* __restrict__ visits, int64_t s,
     int64_t win, const int64_t* __restrict__ map, const uint8_t* __restrict__ flags,
     const double2* __restrict__ phases, const void* __restrict__ B, int is_f32, double2* __restrict__ S,
     int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt, PendingRec* __restrict__ rec, int cap,
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const in
 {
     __shared__ double red_re[kBulkThreads / 64], red_im[kBulkThreads / 64];
     __shared__ int red_n[kBulkThreads / 64], red_e[kBulkThreads / 64], red_p[kBulkThreads / 64];
-    const int j = blockIdx.x;
+    const int j = bx;
     const int tid = threadIdx.x;
     if (status[0])
         return; // an earlier window raised
@@ -443,7 +444,7 @@ __device__ __forceinline__ void fix_stage_chunk(int4* dst, const int4* __restric
     }
 }
 
-__global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
+__device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
@@ -600,6 +601,82 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
         atomicAdd(&status[1], calls);
         atomicAdd(&status[2], flagged);
     }
+}
+
+__global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __restrict__ first_pos, RecoArgs a,
+    int64_t s, int64_t e, int64_t win, int64_t* __restrict__ map)
+{
+    reco_map_body(blockIdx.x, visits, first_pos, a, s, e, win, map);
+}
+
+__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits, int64_t s,
+    int64_t win, const int64_t* __restrict__ map, const uint8_t* __restrict__ flags,
+    const double2* __restrict__ phases, const void* __restrict__ B, int is_f32, double2* __restrict__ S,
+    int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt, PendingRec* __restrict__ rec, int cap,
+    const int32_t* __restrict__ status)
+{
+    reco_bulk_body(blockIdx.x, a, visits, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec, cap, status);
+}
+
+__global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
+    const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
+    const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
+    const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
+    double* __restrict__ cons, int32_t* __restrict__ status)
+{
+    reco_fixup_body(a, visits, first_pos, s, e, S, N, E, cnt, rec, cap, flags, phases, cons, status);
+}
+
+// ---------------------------------------------------------------------------
+// many recoveries at once: the same bodies over a table of jobs, one launch per
+// stage and window index for all of them (jobs without that window return)
+// ---------------------------------------------------------------------------
+struct RecoJob {
+    RecoArgs a;
+    const int2* visits;
+    const int32_t* first_pos;
+    int64_t* map;
+    double2* S;
+    int32_t *N, *E, *cnt;
+    PendingRec* rec;
+    int32_t* status;
+    const void* B;
+    uint8_t* flags;
+    double2* phases;
+    double* cons;
+    int32_t cap, is_f32;
+};
+
+__global__ void k_reco_map_many(const RecoJob* __restrict__ jobs, int64_t win)
+{
+    const RecoJob& J = jobs[blockIdx.y];
+    const RecoArgs a = J.a;
+    const int64_t s0 = win * kWindow;
+    if (s0 >= a.nvisits)
+        return;
+    reco_map_body(blockIdx.x, J.visits, J.first_pos, a, s0, min(s0 + kWindow, a.nvisits), win, J.map);
+}
+
+__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* __restrict__ jobs, int64_t win)
+{
+    const RecoJob& J = jobs[blockIdx.y];
+    const RecoArgs a = J.a;
+    const int64_t s0 = win * kWindow;
+    if (s0 + (int64_t)blockIdx.x >= a.nvisits)
+        return;
+    reco_bulk_body(blockIdx.x, a, J.visits, s0, win, J.map, J.flags, J.phases, J.B, J.is_f32, J.S, J.N, J.E, J.cnt,
+        J.rec, J.cap, J.status);
+}
+
+__global__ __launch_bounds__(64) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
+{
+    const RecoJob& J = jobs[blockIdx.x];
+    const RecoArgs a = J.a;
+    const int64_t s0 = win * kWindow;
+    if (s0 >= a.nvisits)
+        return;
+    reco_fixup_body(a, J.visits, J.first_pos, s0, min(s0 + kWindow, a.nvisits), J.S, J.N, J.E, J.cnt, J.rec, J.cap,
+        J.flags, J.phases, J.cons, J.status);
 }
 
 // ---------------------------------------------------------------------------
@@ -848,6 +925,68 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
     if (st[0])
         fail(st[0], "reconstruct_phases: bispectrum element address out of bounds");
     return st[1];
+}
+
+// Phase recovery of n bispectra at once: per window index one map, one bulk and
+// one fixup launch cover every job that has that window, so the single-wave
+// fixups of different jobs run side by side.  Each job's arithmetic and order
+// are those of run_reconstruct.  calls[k] receives job k's calc_phase count and
+// status[k] its status (SMIP_ERR_OUT_OF_RANGE where get_element would throw).
+void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const int* is_f32,
+    double2* const* phases, uint8_t* const* flags, double* const* consistency, int32_t* calls, int32_t* status,
+    hipStream_t s)
+{
+    if (n <= 0)
+        return;
+    std::vector<RecoJob> tab(n);
+    int64_t maxwin = 0;
+    for (int k = 0; k < n; ++k) {
+        const RecoArgs& a = w[k]->args;
+        const int npts = a.X * a.Y;
+        hipLaunchKernelGGL(k_reco_init, dim3((npts + 255) / 256), dim3(256), 0, s, a, phases[k], flags[k],
+            consistency[k], w[k]->map);
+        hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w[k]->status);
+        RecoJob& J = tab[k];
+        J.a = a;
+        J.visits = w[k]->visits;
+        J.first_pos = w[k]->first_pos;
+        J.map = w[k]->map;
+        J.S = w[k]->S;
+        J.N = w[k]->N;
+        J.E = w[k]->E;
+        J.cnt = w[k]->cnt;
+        J.rec = w[k]->rec;
+        J.status = w[k]->status;
+        J.B = B[k];
+        J.flags = flags[k];
+        J.phases = phases[k];
+        J.cons = consistency[k];
+        J.cap = w[k]->cap;
+        J.is_f32 = is_f32[k];
+        maxwin = std::max(maxwin, w[k]->nwin);
+    }
+    SMIP_HIP(hipGetLastError());
+    RecoJob* dtab = static_cast<RecoJob*>(dev_malloc(sizeof(RecoJob) * n));
+    try {
+        copy_sync(dtab, tab.data(), sizeof(RecoJob) * n, hipMemcpyHostToDevice, s);
+        for (int64_t win = 0; win < maxwin; ++win) {
+            hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
+            hipLaunchKernelGGL(k_reco_bulk_many, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
+            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(64), 0, s, dtab, win);
+        }
+        SMIP_HIP(hipGetLastError());
+        std::vector<int32_t> st(4);
+        for (int k = 0; k < n; ++k) {
+            SMIP_HIP(hipMemcpyAsync(st.data(), w[k]->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+            SMIP_HIP(hipStreamSynchronize(s));
+            status[k] = st[0]; // per job: an out-of-range lookup stops that job only
+            calls[k] = st[1];
+        }
+    } catch (...) {
+        dev_free(dtab);
+        throw;
+    }
+    dev_free(dtab);
 }
 
 // One calc_phase(w) on caller-provided state (include/phasereco.h:75-123): the

@@ -67,6 +67,7 @@ struct smip_server {
     size_t rr = 0;                // round-robin position in `active`
     uint64_t next_id = 1, in_use = 0;
     uint64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_failed = 0, frames = 0;
+    uint64_t n_groups = 0, n_grouped = 0; // finalisation groups and the jobs they held
     std::vector<std::thread> workers;
 
     // first-fit admission in arrival order; caller holds m
@@ -148,6 +149,19 @@ struct smip_server {
             std::vector<uint8_t> chunk;
             const bool create = j->job == nullptr;
             const bool finalize = !create && j->chunks.empty() && j->closed;
+            // continuous batching of phase recovery: every job ready to finalise now
+            // joins this one, and their recursions run as one group
+            std::vector<ServedJob*> group;
+            if (finalize) {
+                group.push_back(j);
+                for (uint64_t id : active) {
+                    ServedJob* o = jobs.at(id).get();
+                    if (o != j && !o->busy && o->state == SMIP_JOB_ACTIVE && o->job && o->closed && o->chunks.empty()) {
+                        o->busy = true;
+                        group.push_back(o);
+                    }
+                }
+            }
             if (!create && !finalize) {
                 chunk = std::move(j->chunks.front());
                 j->chunks.pop_front();
@@ -156,13 +170,19 @@ struct smip_server {
             int rc = SMIP_OK;
             std::string err;
             uint64_t nf = 0;
+            std::vector<int32_t> gst(group.size(), SMIP_OK);
             if (create) {
                 smip_job* job = nullptr;
                 rc = smip_job_create(ctx, &j->p, &job);
                 if (rc == SMIP_OK)
                     j->job = job;
             } else if (finalize) {
-                rc = smip_job_finalize(j->job);
+                std::vector<smip_job*> handles;
+                for (ServedJob* g : group)
+                    handles.push_back(g->job);
+                rc = smip_job_finalize_many(handles.data(), handles.size(), gst.data());
+                if (rc != SMIP_OK && std::all_of(gst.begin(), gst.end(), [](int32_t c) { return c == SMIP_OK; }))
+                    std::fill(gst.begin(), gst.end(), rc); // the group as a whole failed
             } else {
                 nf = chunk.size() / j->frame_bytes;
                 rc = smip_job_push_frames_host(j->job, chunk.data(), nf);
@@ -171,17 +191,30 @@ struct smip_server {
                 err = smip_last_error();
             chunk.clear();
             lk.lock();
-            j->busy = false;
-            if (rc != SMIP_OK) {
-                finish_locked(j, SMIP_JOB_FAILED, err);
-            } else if (finalize) {
-                finish_locked(j, SMIP_JOB_DONE, "");
+            if (finalize) {
+                n_groups += 1;
+                n_grouped += group.size();
+                for (size_t g = 0; g < group.size(); ++g) {
+                    ServedJob* o = group[g];
+                    o->busy = false;
+                    if (gst[g] == SMIP_OK)
+                        finish_locked(o, SMIP_JOB_DONE, "");
+                    else
+                        finish_locked(o, SMIP_JOB_FAILED, err);
+                    if (o->released)
+                        destroy_locked(o);
+                }
             } else {
-                j->frames_done += nf;
-                frames += nf;
+                j->busy = false;
+                if (rc != SMIP_OK) {
+                    finish_locked(j, SMIP_JOB_FAILED, err);
+                } else if (!create) {
+                    j->frames_done += nf;
+                    frames += nf;
+                }
+                if (j->released)
+                    destroy_locked(j);
             }
-            if (j->released)
-                destroy_locked(j);
             cv_work.notify_all();
         }
     }
@@ -406,5 +439,7 @@ extern "C" SMIP_API int smip_server_get_stats(smip_server* s, smip_server_stats*
         out->bytes_in_use = s->in_use;
         out->budget = s->budget;
         out->frames = s->frames;
+        out->finalize_groups = s->n_groups;
+        out->finalized_in_groups = s->n_grouped;
     });
 }

@@ -259,6 +259,11 @@ struct RecoWork;
 RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits,
     hipStream_t stream);
 void reco_work_destroy(RecoWork*);
+// n recoveries together (one launch per stage and window index); calls[k] = job k's
+// calc_phase count, status[k] = its status
+void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const int* is_f32,
+    double2* const* phases, uint8_t* const* flags, double* const* consistency, int32_t* calls, int32_t* status,
+    hipStream_t s);
 void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s);
 int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,

@@ -77,7 +77,8 @@ class ServerJobInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
                 ("failed", C.c_uint64), ("waiting", C.c_uint64), ("active", C.c_uint64),
-                ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64), ("frames", C.c_uint64)]
+                ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64), ("frames", C.c_uint64),
+                ("finalize_groups", C.c_uint64), ("finalized_in_groups", C.c_uint64)]
 
 
 class TuneResult(C.Structure):
@@ -131,6 +132,7 @@ _SIGS = {
     "smip_job_add_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_push_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_finalize": (C.c_int, [_P]),
+    "smip_job_finalize_many": (C.c_int, [_P, C.c_uint64, _P]),
     "smip_job_synchronize": (C.c_int, [_P]),
     "smip_job_get_stats": (C.c_int, [_P, _P]),
     "smip_job_bispectrum": (C.c_int, [_P, _P]),
@@ -427,6 +429,13 @@ def job_params(ncols, nrows, bdepth, reco_radius, frame_dtype, precision=PRECISI
                frames_per_batch=0, vy_tile=0) -> JobParams:
     return JobParams(ncols, nrows, bdepth, float(reco_radius), _NP_DTYPES[np.dtype(frame_dtype)], precision,
                      int(bool(saa)), frames_per_batch, vy_tile)
+
+
+def finalize_many(jobs: list) -> None:
+    """Job.finalize for several jobs at once; their phase recoveries run as one group."""
+    arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+    st = (C.c_int32 * len(jobs))()
+    _check(lib().smip_job_finalize_many(arr, len(jobs), st))
 
 
 def resolve_params(params: JobParams) -> JobParams:

@@ -188,6 +188,7 @@ def test_result_is_the_same_alone_and_in_the_mix(ctx):
         srv.release(jid)
     st = srv.stats()
     assert st["completed"] == len(data) and st["failed"] == 0 and st["bytes_in_use"] == 0
+    assert st["finalized_in_groups"] == len(data) and 1 <= st["finalize_groups"] <= len(data)
     srv.close()
 
 
@@ -281,3 +282,51 @@ def test_captures_concurrent_with_allocation_and_copies(ctx):
         t.join()
     assert not errors, errors[:3]
     assert not bad, bad
+
+
+def test_finalize_many_matches_single(ctx):
+    """Grouped phase recovery (one launch per stage and window index for all jobs)
+    gives each job exactly what its own finalize gives."""
+    spec = [("binary128", 64, 30, 16, 32), ("triple256", 128, 20, 32, 64), ("extended512", 64, 25, 20, 40),
+            ("binary128", 32, 40, 12, 16), ("triple256", 128, 18, 24, 24)]
+    data = [(_frames(kind, N, n, seed_offset=40 + i), S.job_params(N, N, b, p, np.dtype(
+        {"binary128": np.uint8, "triple256": np.uint16, "extended512": np.float32}[kind]), frames_per_batch=16))
+        for i, (kind, N, n, b, p) in enumerate(spec)]
+    single, calls = [], []
+    for fr, prm in data:
+        job = S.Job(ctx, prm)
+        job.add_frames(S.Frames(ctx, fr))
+        job.finalize()
+        single.append(_outputs(job))
+        calls.append(job.stats()["reco_calls"])
+        job.close()
+    jobs = []
+    for fr, prm in data:
+        job = S.Job(ctx, prm)
+        job.push_frames_host(fr)
+        jobs.append(job)
+    S.finalize_many(jobs)
+    for k, job in enumerate(jobs):
+        _assert_identical(_outputs(job), single[k])
+        assert job.stats()["reco_calls"] == calls[k]
+        job.close()
+
+
+def test_finalize_many_rules(ctx):
+    fr = _frames("binary128", 64, 8)
+    prm = S.job_params(64, 64, 16, 16, fr.dtype)
+    a, b = S.Job(ctx, prm), S.Job(ctx, prm)
+    a.push_frames_host(fr)
+    S.finalize_many([])  # nothing to do
+    with pytest.raises(S.SmipError) as e:
+        S.finalize_many([a, a])
+    assert e.value.code == S.ERR_INVALID_ARGUMENT
+    with pytest.raises(S.SmipError) as e:
+        S.finalize_many([a, b])  # b has no frames: nothing is changed
+    assert e.value.code == S.ERR_RUNTIME
+    S.finalize_many([a])
+    with pytest.raises(S.SmipError) as e:
+        S.finalize_many([a])
+    assert e.value.code == S.ERR_LOGIC
+    a.close()
+    b.close()
