@@ -37,8 +37,11 @@ VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (dense)
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg: str) -> None:
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench +{time.perf_counter() - _T0:.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def traffic_from_profile(path: str):
@@ -54,9 +57,11 @@ def traffic_from_profile(path: str):
     return None
 
 
-def cpu_baseline(frames: np.ndarray, b: int, seconds: float) -> dict:
+def cpu_baseline(frames: np.ndarray, b: int, p: float, seconds: float) -> dict:
     """The reference's per-frame CPU path (FP64 FFT, complex<float> accumulate,
-    smip-cli/main.cpp:256-305) as restated by the oracle, single thread."""
+    smip-cli/main.cpp:256-305) as restated by the oracle, single thread; then
+    its phase recovery (include/phasereco.h:33-123) once on the sample's
+    normalised bispectrum."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O  # the checker, used here only as the CPU baseline
 
@@ -66,11 +71,96 @@ def cpu_baseline(frames: np.ndarray, b: int, seconds: float) -> dict:
     one = time.perf_counter() - t0
     k = int(max(1, min(len(f64) - 1, seconds / max(one, 1e-3))))
     t0 = time.perf_counter()
-    O.power_and_bispectrum(f64[1:1 + k], b, bispec_f32=True, fast=True)
+    B, _ = O.power_and_bispectrum(f64[1:1 + k], b, bispec_f32=True, fast=True)
     dt = time.perf_counter() - t0
+    N = frames.shape[-1]
+    B /= np.complex64(k)
+    t0 = time.perf_counter()
+    reco = O.reconstruct_phases(O.desc(N, N, b), B, N, N, p)
+    dreco = time.perf_counter() - t0
     return {"value": k / dt, "unit": "frames/s", "cores": 1, "kind": "port",
             "sample": f"{k} of the workload's frames through the oracle's FP64 FFT + complex<float> "
-                      f"accumulate (-O3 -ffast-math), 1 thread, {dt:.1f} s on {platform.processor() or 'host'}"}
+                      f"accumulate (-O3 -ffast-math), 1 thread, {dt:.1f} s on {platform.processor() or 'host'}",
+            "phase_recovery_s": round(dreco, 3), "phase_recovery_calls": int(reco.stats["calls"]),
+            "phase_recovery_note": "the oracle's calc_phase over the (2h+1)^2 candidates of each visit "
+                                   "(the reference scans all N^2 u per call), 1 thread"}
+
+
+def streamed_leg(ctx, frames: np.ndarray, params) -> dict:
+    """The metric's accumulate with the frames in pageable host memory, so the
+    PCIe copy is inside the timed region: one smip_job_add_frames_host call for
+    the stack, and smip_job_push_frames_host in camera-sized chunks of 100."""
+    batch = S.resolve_params(params).frames_per_batch
+    out = {"frames": len(frames), "frames_per_batch": batch, "host_memory": "pageable (numpy)"}
+    j = S.Job(ctx, params)
+    j.add_frames_host(frames[:batch])  # allocations
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    j.add_frames_host(frames)
+    ctx.synchronize()
+    out["add_frames_host_frames_per_s"] = round(len(frames) / (time.perf_counter() - t0), 1)
+    j.close()
+    def push(job, count, chunk=100):  # count frames, cycling through the stack
+        done = 0
+        while done < count:
+            s0 = done % len(frames)
+            c = min(chunk, count - done, len(frames) - s0)
+            job.push_frames_host(frames[s0:s0 + c])
+            done += c
+
+    whole = max(1, len(frames) // batch) * batch  # whole batches: nothing left pending
+    j = S.Job(ctx, params)
+    push(j, 2 * batch)  # the eager first batch and the graph capture
+    j.synchronize()
+    t0 = time.perf_counter()
+    push(j, whole)
+    j.synchronize()
+    out["push_frames_host_frames_per_s"] = round(whole / (time.perf_counter() - t0), 1)
+    out["push_frames_timed"] = whole
+    out["push_chunk"] = 100
+    j.close()
+    return out
+
+
+def workload_leg(ctx, name: str, nframes: int, passes: int, latency_runs: int) -> dict:
+    """Another configuration of the reference's: accumulate frames/s on a
+    device-resident sample of its stack (mixed path), and the latency of complete
+    jobs (accumulate, shift-and-add, phases, synthesis) on that sample."""
+    W = S.WORKLOADS[name]
+    n = min(nframes, W["n"])
+    fr = S.synth_frames(S.synth_params(name), 0, n, threads=min(16, os.cpu_count() or 1))
+    dev = S.Frames(ctx, fr)
+    p = S.job_params(W["N"], W["N"], W["b"], W["p"], W["dtype"], S.PRECISION_MIXED, saa=False)
+    job = S.Job(ctx, p)
+    job.add_frames(dev)  # warm-up
+    ctx.synchronize()
+    st0 = job.stats()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        job.add_frames(dev)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st1 = job.stats()
+    job.close()
+    tile_s = (st1["tile_ms"] - st0["tile_ms"]) * 1e-3
+    flops = 14.0 * st1["triples_per_frame"] * n * passes
+    lat = []
+    stages = None
+    for _ in range(latency_runs):
+        t = time.perf_counter()
+        jl = S.Job(ctx, S.job_params(W["N"], W["N"], W["b"], W["p"], W["dtype"], S.PRECISION_MIXED, saa=True))
+        jl.add_frames(dev)
+        jl.finalize()
+        lat.append((time.perf_counter() - t) * 1e3)
+        stages = jl.stats()
+        jl.close()
+    return {"workload": f"{name}: {n} of {W['n']} frames {W['N']}x{W['N']} {np.dtype(W['dtype']).name}, "
+                        f"b={W['b']}, p={W['p']}",
+            "frames_per_s": round(n * passes / dt, 1), "frames_timed": n * passes,
+            "tile_tflops": round(flops / tile_s / 1e12, 2) if tile_s > 0 else None,
+            "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "runs": latency_runs,
+                               "frames_per_job": n},
+            "stages_ms": {k: round(stages[k], 2) for k in ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}}
 
 
 def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int):
@@ -431,7 +521,10 @@ def main() -> int:
     ap.add_argument("--batch-seqs", type=int, default=64, help="sequences of the multi-sequence leg (0: skip)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues per process for HIP (GPU_MAX_HW_QUEUES, at most 32; 0: HIP's default)")
-    ap.add_argument("--only", default="", help="comma list of legs to run: latency,batch,precision,serve,cpu")
+    ap.add_argument("--c3-frames", type=int, default=512, help="frames of the 512x512 throughput sample")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 128x128 and 512x512 legs")
+    ap.add_argument("--only", default="",
+                    help="comma list of legs to run: latency,configs,batch,precision,serve,cpu")
     ap.add_argument("--precision-frames", type=int, default=128,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
@@ -440,9 +533,11 @@ def main() -> int:
             print(json.dumps({"error": "--hw-queues must be in [1, 32]"}))
             return 2
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)  # read when HIP initialises (first context)
+    args.other_configs = not args.no_configs
     if args.only:
         legs = set(args.only.split(","))
         args.no_latency = "latency" not in legs
+        args.other_configs = "configs" in legs
         args.no_cpu = "cpu" not in legs
         args.batch_seqs = args.batch_seqs if "batch" in legs else 0
         args.precision_frames = args.precision_frames if "precision" in legs else 0
@@ -507,6 +602,10 @@ def main() -> int:
     j64.close()
     log(f"plain FP64: {fps64:.0f} frames/s over {n64} frames; speedup {fps / fps64:.1f}x")
 
+    log("the same accumulate streamed from host memory")
+    streamed = streamed_leg(ctx, frames, p)
+    log(f"streamed from host: {streamed}")
+
     out = {
         "metric": "bispectrum frames/s @256^2,b=48,p=128 vs plain FP64 GPU path; p50/p99 job latency",
         "value": round(fps, 1),
@@ -532,6 +631,7 @@ def main() -> int:
                      "traffic": traffic_from_profile(os.path.join(REPO, "profiles", "tile_triple256_pmc.txt")),
                      "kernel": "k_accum_tile", "flops_per_launch": flops_per_launch,
                      "avg_launch_ms": round(tile_ms / launches, 4)},
+        "streamed_from_host": streamed,
     }
 
     # ---- complete jobs: latency and accuracy ---------------------------------
@@ -569,6 +669,13 @@ def main() -> int:
         }
         log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
 
+    if args.other_configs:
+        log("the other single-sequence configurations")
+        out["binary128"] = workload_leg(ctx, "binary128", 40, passes=50, latency_runs=5)
+        log(f"binary128: {out['binary128']}")
+        out["extended512"] = workload_leg(ctx, "extended512", args.c3_frames, passes=2, latency_runs=1)
+        log(f"extended512: {out['extended512']}")
+
     if args.batch_seqs > 0:
         log(f"multi-sequence: {args.batch_seqs} sequences, 16 host threads")
         out["multi_sequence"] = batch_leg(ctx, args.batch_seqs, 16, plain_sample=32)
@@ -588,7 +695,7 @@ def main() -> int:
 
     if not args.no_cpu:
         log("timing the CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(frames, b, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(frames, b, pr, args.cpu_seconds)
         log(f"cpu baseline {out['cpu_baseline']['value']:.3f} frames/s")
 
     ctx.close()

@@ -12,9 +12,11 @@
 #include <cstring>
 #include <memory>
 #include <tuple>
+#include <condition_variable>
 #include <mutex>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "smip_internal.h"
@@ -209,18 +211,26 @@ struct smip_job {
     dptr<double> corr, sum;
     dptr<int32_t> shifts;
     uint64_t shifts_cap = 0;
-    // host upload staging
-    dptr<uint8_t> upload;
-    // buffered streaming (smip_job_push_frames_host): frames collect in `upload`
-    // until a full batch; host data passes through two pinned slots
-    uint64_t staged = 0;
+    // frames from host memory: two device staging buffers of one batch each.  The
+    // copy into one runs on `ustream` while the batch in the other is processed
+    // on `stream`; ev_up[k] marks buffer k's copies done, ev_used[k] its batch done.
+    dptr<uint8_t> upload[2];
+    hipStream_t ustream = nullptr;
+    hipEvent_t ev_up[2] = { nullptr, nullptr }, ev_used[2] = { nullptr, nullptr };
+    int up_cur = 0;
+    // host data reaches the device through two pinned slots; pin_done[k] marks
+    // slot k's copy to the device done
     uint8_t* pin[2] = { nullptr, nullptr };
     hipEvent_t pin_done[2] = { nullptr, nullptr };
-    uint64_t pin_frames = 0; // frames per pinned slot
+    size_t pin_bytes = 0;
     int pin_cur = 0;
-    // per-batch launch sequences captured as HIP graphs (full batches from the
-    // upload buffer): accumulate, and shift-and-add into a fixed shift scratch
-    hipGraphExec_t g_acc = nullptr, g_saa = nullptr;
+    // buffered streaming (smip_job_push_frames_host): frames collect in
+    // upload[up_cur] until a full batch
+    uint64_t staged = 0;
+    // per-batch launch sequences captured as HIP graphs (full batches from a
+    // staging buffer, one graph each): accumulate, and shift-and-add into a
+    // fixed shift scratch
+    hipGraphExec_t g_acc[2] = { nullptr, nullptr }, g_saa[2] = { nullptr, nullptr };
     dptr<int32_t> bshift;
     uint64_t graph_launches = 0;
     // device-stack path: the forward FFT of batch k+1 runs on fstream into the
@@ -252,16 +262,16 @@ struct smip_job {
         }
         destroy_stream(fstream);
         destroy_stream(sstream);
-        for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork, ev_sjoin })
+        destroy_stream(ustream);
+        for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork, ev_sjoin, ev_up[0], ev_up[1],
+                 ev_used[0], ev_used[1], pin_done[0], pin_done[1] })
             if (e)
                 (void)hipEventDestroy(e);
-        if (g_acc)
-            (void)hipGraphExecDestroy(g_acc);
-        if (g_saa)
-            (void)hipGraphExecDestroy(g_saa);
         for (int k = 0; k < 2; ++k) {
-            if (pin_done[k])
-                (void)hipEventDestroy(pin_done[k]);
+            if (g_acc[k])
+                (void)hipGraphExecDestroy(g_acc[k]);
+            if (g_saa[k])
+                (void)hipGraphExecDestroy(g_saa[k]);
             host_free(pin[k]);
         }
         tile_plan_destroy(tile);
@@ -1100,7 +1110,7 @@ JobSizes job_sizes(const smip_job_params& p)
         b += hx * Y * sizeof(double2)
             + (uint64_t)std::min(z.batch, kSaaChunk) * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
             + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
-    b += (uint64_t)z.batch * np * dtype_size(p.frame_dtype);             // host upload staging
+    b += 2 * (uint64_t)z.batch * np * dtype_size(p.frame_dtype);         // two host staging buffers
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
     z.bytes = b;
     return z;
@@ -1216,6 +1226,15 @@ void enable_overlap(smip_job* j)
     }
 }
 
+// which staging buffer `frames` is (-1: none, frames from a device stack)
+int upload_index(const smip_job* j, const void* frames)
+{
+    for (int k = 0; k < 2; ++k)
+        if (j->upload[k] && frames == j->upload[k].get())
+            return k;
+    return -1;
+}
+
 void job_process(smip_job* j, const void* frames, uint64_t count)
 {
     const smip_job_params& p = j->p;
@@ -1224,7 +1243,7 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     hipStream_t s = j->stream;
     const bool mixed = p.precision == SMIP_PRECISION_MIXED;
     // frames from a device stack (not the reused upload buffer): FFT / tile overlap
-    const bool overlap = mixed && count > (uint64_t)j->batch && !(j->upload && frames == j->upload.get());
+    const bool overlap = mixed && count > (uint64_t)j->batch && upload_index(j, frames) < 0;
     hipStream_t ss = s; // stream of the shift-and-add
     if (overlap) {
         enable_overlap(j);
@@ -1239,14 +1258,15 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     for (uint64_t off = 0; off < count; ++k) {
         const int64_t nf = (int64_t)std::min<uint64_t>((uint64_t)j->batch, count - off);
         const uint8_t* fr = static_cast<const uint8_t*>(frames) + off * fb;
-        // full batches from the upload buffer after frame 0 read and write the same
+        // full batches from a staging buffer after frame 0 read and write the same
         // buffers every time: one graph launch each instead of a dozen launches
-        const bool graph = graphs_enabled() && nf == j->batch && j->upload && fr == j->upload.get() && j->nframes > 0;
+        const int ub = upload_index(j, fr);
+        const bool graph = graphs_enabled() && nf == j->batch && ub >= 0 && j->nframes > 0;
         hipEvent_t e0 = mark(j);
         if (graph) {
-            if (!j->g_acc)
-                j->g_acc = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
-            SMIP_HIP(hipGraphLaunch(j->g_acc, s));
+            if (!j->g_acc[ub])
+                j->g_acc[ub] = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
+            SMIP_HIP(hipGraphLaunch(j->g_acc[ub], s));
             j->graph_launches++;
         } else if (overlap) {
             const int b = (int)(k & 1);
@@ -1279,9 +1299,9 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
         if (p.saa) {
             hipEvent_t e2 = mark_on(j, ss);
             if (graph) {
-                if (!j->g_saa)
-                    j->g_saa = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get(), s); });
-                SMIP_HIP(hipGraphLaunch(j->g_saa, s));
+                if (!j->g_saa[ub])
+                    j->g_saa[ub] = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get(), s); });
+                SMIP_HIP(hipGraphLaunch(j->g_saa[ub], s));
                 j->graph_launches++;
                 SMIP_HIP(hipMemcpyAsync(j->shifts.get() + 2 * j->nframes, j->bshift.get(), sizeof(int32_t) * 2 * nf,
                     hipMemcpyDeviceToDevice, s));
@@ -1443,6 +1463,138 @@ extern "C" SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, u
     });
 }
 
+// Copies from caller memory into pinned memory, split over a few helper threads
+// so that this host-side step keeps ahead of the kernels it feeds (one thread
+// moves pageable memory at roughly a fifth of the PCIe rate).  The pool is
+// never destroyed: its helpers may still be waiting when the process exits.
+class CopyPool {
+public:
+    static CopyPool& get()
+    {
+        static CopyPool* pool = new CopyPool();
+        return *pool;
+    }
+    void copy(void* dst, const void* src, size_t bytes)
+    {
+        if (helpers_ == 0 || bytes < ((size_t)1 << 20)) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        std::lock_guard<std::mutex> one(use_); // one split copy at a time
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            bytes_ = bytes;
+            pending_ = helpers_;
+            ++gen_;
+        }
+        work_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        helpers_ = hw >= 4 ? (int)std::min<unsigned>(7, hw / 2 - 1) : 0;
+        for (int t = 1; t <= helpers_; ++t)
+            std::thread([this, t] { run(t); }).detach();
+    }
+    void part(int i) const
+    {
+        const size_t per = ((bytes_ / (size_t)(helpers_ + 1)) + 63) & ~(size_t)63;
+        const size_t a = std::min(bytes_, per * (size_t)i);
+        const size_t b = i == helpers_ ? bytes_ : std::min(bytes_, a + per);
+        if (b > a)
+            std::memcpy(dst_ + a, src_ + a, b - a);
+    }
+    void run(int i)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(m_);
+            work_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            lk.unlock();
+            part(i);
+            lk.lock();
+            if (--pending_ == 0)
+                done_.notify_one();
+        }
+    }
+    std::mutex use_, m_;
+    std::condition_variable work_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    int helpers_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+};
+
+// Host frames enter through the two staging buffers: ingest_copy moves host data
+// through the pinned slots into buffer k on the copy stream, once k's previous
+// batch is done with it; ingest_process runs the batch in k on the job stream
+// once its copies landed.  The copies of one batch overlap the processing of
+// the other, and the caller's memory is free as soon as ingest_copy returns.
+constexpr size_t kPinSlotBytes = (size_t)8 << 20;
+
+void ensure_ingest(smip_job* j)
+{
+    if (j->ustream)
+        return;
+    const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
+    for (int k = 0; k < 2; ++k) {
+        j->upload[k] = dmake<uint8_t>((size_t)j->batch * fb);
+        SMIP_HIP(hipEventCreateWithFlags(&j->ev_up[k], hipEventDisableTiming));
+        SMIP_HIP(hipEventCreateWithFlags(&j->ev_used[k], hipEventDisableTiming));
+        SMIP_HIP(hipEventRecord(j->ev_used[k], j->stream));
+    }
+    j->pin_bytes = std::min(kPinSlotBytes, (size_t)j->batch * fb);
+    for (int k = 0; k < 2; ++k) {
+        j->pin[k] = static_cast<uint8_t*>(host_malloc(j->pin_bytes));
+        SMIP_HIP(hipEventCreateWithFlags(&j->pin_done[k], hipEventDisableTiming));
+        SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
+    }
+    j->ustream = make_stream();
+}
+
+void ingest_copy(smip_job* j, int k, uint64_t at, const uint8_t* src, uint64_t n, size_t fb)
+{
+    SMIP_HIP(hipStreamWaitEvent(j->ustream, j->ev_used[k], 0));
+    const size_t bytes = n * fb;
+    uint8_t* dst = j->upload[k].get() + at * fb;
+    for (size_t o = 0; o < bytes; o += j->pin_bytes) {
+        const size_t len = std::min(j->pin_bytes, bytes - o);
+        const int sl = j->pin_cur;
+        j->pin_cur ^= 1;
+        SMIP_HIP(hipEventSynchronize(j->pin_done[sl])); // the slot's previous piece has left it
+        CopyPool::get().copy(j->pin[sl], src + o, len);
+        SMIP_HIP(hipMemcpyAsync(dst + o, j->pin[sl], len, hipMemcpyHostToDevice, j->ustream));
+        SMIP_HIP(hipEventRecord(j->pin_done[sl], j->ustream));
+    }
+}
+
+void ingest_process(smip_job* j, int k, uint64_t nf)
+{
+    SMIP_HIP(hipEventRecord(j->ev_up[k], j->ustream));
+    SMIP_HIP(hipStreamWaitEvent(j->stream, j->ev_up[k], 0));
+    job_process(j, j->upload[k].get(), nf);
+    SMIP_HIP(hipEventRecord(j->ev_used[k], j->stream));
+}
+
+// the batch collected by smip_job_push_frames_host, if any
+void flush_staged(smip_job* j)
+{
+    if (j->staged == 0)
+        return;
+    ingest_process(j, j->up_cur, j->staged);
+    j->up_cur ^= 1;
+    j->staged = 0;
+}
+
 extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t count)
 {
     return guarded([&] {
@@ -1451,15 +1603,15 @@ extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames
         require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
         ensure_device(j->ctx);
         grow_shifts(j, j->nframes + count);
+        ensure_ingest(j);
         const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
-        if (!j->upload)
-            j->upload = dmake<uint8_t>((size_t)j->batch * fb);
+        const uint8_t* src = static_cast<const uint8_t*>(frames);
         for (uint64_t off = 0; off < count;) {
             const uint64_t nf = std::min<uint64_t>((uint64_t)j->batch, count - off);
-            SMIP_HIP(hipStreamSynchronize(j->stream)); // staging is reused
-            SMIP_HIP(hipMemcpyAsync(j->upload.get(), static_cast<const uint8_t*>(frames) + off * fb, nf * fb,
-                hipMemcpyHostToDevice, j->stream));
-            job_process(j, j->upload.get(), nf);
+            const int k = j->up_cur;
+            j->up_cur ^= 1;
+            ingest_copy(j, k, 0, src + off * fb, nf, fb);
+            ingest_process(j, k, nf);
             off += nf;
         }
         SMIP_HIP(hipStreamSynchronize(j->stream));
@@ -1472,38 +1624,19 @@ extern "C" SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frame
         require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
         ensure_device(j->ctx);
+        ensure_ingest(j);
         const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
         const uint64_t batch = (uint64_t)j->batch;
-        if (!j->upload)
-            j->upload = dmake<uint8_t>((size_t)batch * fb);
-        if (!j->pin[0]) {
-            // pinned slots of at most 8 MiB (at least one frame), never more than a batch
-            j->pin_frames = std::max<uint64_t>(1, std::min<uint64_t>(batch, (8u << 20) / fb));
-            for (int k = 0; k < 2; ++k) {
-                j->pin[k] = static_cast<uint8_t*>(host_malloc(j->pin_frames * fb));
-                SMIP_HIP(hipEventCreateWithFlags(&j->pin_done[k], hipEventDisableTiming));
-                SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
-            }
-        }
         grow_shifts(j, j->nframes + j->staged + count);
         const uint8_t* src = static_cast<const uint8_t*>(frames);
         while (count > 0) {
-            const uint64_t n = std::min({ count, j->pin_frames, batch - j->staged });
-            const int k = j->pin_cur;
-            SMIP_HIP(hipEventSynchronize(j->pin_done[k])); // the slot's previous copy has left it
-            std::memcpy(j->pin[k], src, n * fb);
-            // stream order puts this copy after the kernels that read the previous batch
-            SMIP_HIP(hipMemcpyAsync(j->upload.get() + j->staged * fb, j->pin[k], n * fb, hipMemcpyHostToDevice,
-                j->stream));
-            SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
-            j->pin_cur ^= 1;
+            const uint64_t n = std::min(count, batch - j->staged);
+            ingest_copy(j, j->up_cur, j->staged, src, n, fb);
             j->staged += n;
             src += n * fb;
             count -= n;
-            if (j->staged == batch) {
-                job_process(j, j->upload.get(), batch);
-                j->staged = 0;
-            }
+            if (j->staged == batch)
+                flush_staged(j);
         }
     });
 }
@@ -1514,10 +1647,7 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
         require(j != nullptr, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
         ensure_device(j->ctx);
-        if (j->staged > 0) { // the last, partial batch of pushed frames
-            job_process(j, j->upload.get(), j->staged);
-            j->staged = 0;
-        }
+        flush_staged(j); // the last, partial batch of pushed frames
         require(j->nframes > 0, SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
         hipStream_t s = j->stream;
         const smip_job_params& p = j->p;
@@ -1600,10 +1730,7 @@ extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n
             };
             for (uint64_t k = 0; k < n; ++k) {
                 smip_job* j = jobs[k];
-                if (j->staged > 0) { // the last, partial batch of pushed frames
-                    job_process(j, j->upload.get(), j->staged);
-                    j->staged = 0;
-                }
+                flush_staged(j); // the last, partial batch of pushed frames
                 // bispectrum /= complex(n, 0)
                 launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, j->stream);
                 hipEvent_t e = event(false);
