@@ -233,11 +233,17 @@ struct smip_job {
     hipGraphExec_t g_acc[2] = { nullptr, nullptr }, g_saa[2] = { nullptr, nullptr };
     dptr<int32_t> bshift;
     uint64_t graph_launches = 0;
-    // device-stack path: the forward FFT of batch k+1 runs on fstream into the
-    // other half of a double-buffered spectrum while batch k's triple products run
+    // overlap: the forward FFT of batch k+1 runs on fstream into the other half of
+    // a double-buffered spectrum (H32, H32b) while batch k's triple products run.
+    // Device stacks overlap within a call; host batches of large jobs
+    // (host_overlap) across calls.  hcur is the half the next FFT writes,
+    // hused[b] whether ev_tile[b] marks a pending reader of half b.
     hipStream_t fstream = nullptr;
     dptr<float2> H32b;
     hipEvent_t ev_fft[2] = { nullptr, nullptr }, ev_tile[2] = { nullptr, nullptr }, ev_fork = nullptr;
+    int hcur = 0;
+    bool hused[2] = { false, false };
+    bool host_overlap = false;
     // ... and the shift-and-add runs beside both on its own stream
     hipStream_t sstream = nullptr;
     hipEvent_t ev_sjoin = nullptr;
@@ -260,9 +266,10 @@ struct smip_job {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
+        if (ustream != fstream)
+            destroy_stream(ustream);
         destroy_stream(fstream);
         destroy_stream(sstream);
-        destroy_stream(ustream);
         for (hipEvent_t e : { ev_fft[0], ev_fft[1], ev_tile[0], ev_tile[1], ev_fork, ev_sjoin, ev_up[0], ev_up[1],
                  ev_used[0], ev_used[1], pin_done[0], pin_done[1] })
             if (e)
@@ -1224,6 +1231,21 @@ void enable_overlap(smip_job* j)
         SMIP_HIP(hipEventCreateWithFlags(&j->ev_sjoin, hipEventDisableTiming));
         j->sstream = make_stream();
     }
+    // fstream starts after everything so far on the job stream (padding rows,
+    // earlier batches in H32)
+    SMIP_HIP(hipEventRecord(j->ev_fork, j->stream));
+    SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_fork, 0));
+}
+
+// A batch on the job stream alone used half 0 (H32): a later overlapped FFT
+// into it waits for that batch (before overlap is enabled, its initial fork
+// orders it).
+void note_h32_reader(smip_job* j)
+{
+    if (!j->fstream)
+        return;
+    SMIP_HIP(hipEventRecord(j->ev_tile[0], j->stream));
+    j->hused[0] = true;
 }
 
 // which staging buffer `frames` is (-1: none, frames from a device stack)
@@ -1242,36 +1264,43 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     const int32_t X = (int32_t)p.ncols, Y = (int32_t)p.nrows;
     hipStream_t s = j->stream;
     const bool mixed = p.precision == SMIP_PRECISION_MIXED;
-    // frames from a device stack (not the reused upload buffer): FFT / tile overlap
-    const bool overlap = mixed && count > (uint64_t)j->batch && upload_index(j, frames) < 0;
+    // FFT / tile overlap: device stacks of more than a batch, and host batches
+    // (a staging buffer) of large jobs
+    const int host_buf = upload_index(j, frames);
+    const bool overlap = mixed && (host_buf < 0 ? count > (uint64_t)j->batch : j->host_overlap);
     hipStream_t ss = s; // stream of the shift-and-add
     if (overlap) {
         enable_overlap(j);
-        SMIP_HIP(hipEventRecord(j->ev_fork, s)); // fork: fstream (and sstream) follow everything before this call
-        SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_fork, 0));
+        // fork: sstream follows everything before this call; so does fstream for
+        // a device stack, while for a host batch it needs only the batch's copy
+        // (and, per half, the triple products that last read it), which lets
+        // the FFT run beside the previous call's triple products
+        SMIP_HIP(hipEventRecord(j->ev_fork, s));
+        SMIP_HIP(hipStreamWaitEvent(j->fstream, host_buf < 0 ? j->ev_fork : j->ev_up[host_buf], 0));
         if (p.saa) {
             ss = j->sstream;
             SMIP_HIP(hipStreamWaitEvent(ss, j->ev_fork, 0));
         }
     }
-    int64_t k = 0; // batch index within this call
-    for (uint64_t off = 0; off < count; ++k) {
+    for (uint64_t off = 0; off < count;) {
         const int64_t nf = (int64_t)std::min<uint64_t>((uint64_t)j->batch, count - off);
         const uint8_t* fr = static_cast<const uint8_t*>(frames) + off * fb;
         // full batches from a staging buffer after frame 0 read and write the same
         // buffers every time: one graph launch each instead of a dozen launches
         const int ub = upload_index(j, fr);
-        const bool graph = graphs_enabled() && nf == j->batch && ub >= 0 && j->nframes > 0;
+        const bool graph = graphs_enabled() && !overlap && nf == j->batch && ub >= 0 && j->nframes > 0;
         hipEvent_t e0 = mark(j);
         if (graph) {
             if (!j->g_acc[ub])
                 j->g_acc[ub] = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
             SMIP_HIP(hipGraphLaunch(j->g_acc[ub], s));
             j->graph_launches++;
+            note_h32_reader(j);
         } else if (overlap) {
-            const int b = (int)(k & 1);
+            const int b = j->hcur;
+            j->hcur ^= 1;
             float2* H = b ? j->H32b.get() : j->H32.get();
-            if (k >= 2) // batch k-2's triple products have finished reading H
+            if (j->hused[b]) // the triple products that last read this half have finished
                 SMIP_HIP(hipStreamWaitEvent(j->fstream, j->ev_tile[b], 0));
             launch_forward_f32(j->plan, fr, p.frame_dtype, nf, false, j->hp, H, j->work32.get(), j->P_half.get(),
                 j->fstream);
@@ -1281,6 +1310,7 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
             launch_accum_tile(j->tile, j->hp, H, nf, j->B64.get(), s);
             hipEvent_t t1 = mark(j);
             SMIP_HIP(hipEventRecord(j->ev_tile[b], s));
+            j->hused[b] = true;
             record(j, 4, t0, t1);
             j->tile_launches++;
         } else if (mixed) {
@@ -1289,6 +1319,7 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
             hipEvent_t t0 = mark(j);
             launch_accum_tile(j->tile, j->hp, j->H32.get(), nf, j->B64.get(), s);
             hipEvent_t t1 = mark(j);
+            note_h32_reader(j);
             record(j, 4, t0, t1);
             j->tile_launches++;
         } else {
@@ -1391,6 +1422,9 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         j->batch = z.batch;
         j->footprint = z.bytes;
         j->triples = count_triples(z.L);
+        // host batches of at least ~1 ms of triple products overlap their FFT with
+        // the previous batch; smaller ones replay a captured graph per batch
+        j->host_overlap = p->precision == SMIP_PRECISION_MIXED && 14.0 * (double)j->triples * z.batch > 5e10;
         j->stream = make_stream();
         hipStream_t s = j->stream;
         const uint64_t X = p->ncols, Y = p->nrows, hx = X / 2 + 1, np = X * Y;
@@ -1558,7 +1592,14 @@ void ensure_ingest(smip_job* j)
         SMIP_HIP(hipEventCreateWithFlags(&j->pin_done[k], hipEventDisableTiming));
         SMIP_HIP(hipEventRecord(j->pin_done[k], j->stream));
     }
-    j->ustream = make_stream();
+    // overlapped host batches copy on the FFT stream, right ahead of their FFT
+    // (one stream fewer to share the hardware queues); others on a copy stream
+    if (j->host_overlap) {
+        enable_overlap(j);
+        j->ustream = j->fstream;
+    } else {
+        j->ustream = make_stream();
+    }
 }
 
 void ingest_copy(smip_job* j, int k, uint64_t at, const uint8_t* src, uint64_t n, size_t fb)
