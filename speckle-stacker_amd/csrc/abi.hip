@@ -1179,8 +1179,9 @@ void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh, hipStre
         const int64_t c = std::min<int64_t>(kSaaChunk, m - o);
         const uint8_t* f = fm + o * fb;
         launch_rfft_f64(j->plan, f, p.frame_dtype, c, true, j->sspec.get(), j->swork.get(), s);
-        launch_xcorr_f64(j->plan, j->ref_spec.get(), j->sspec.get(), c, j->corr.get(), j->swork.get(), s);
-        launch_argmax_shift(j->corr.get(), c, X, Y, sh + 2 * o, s);
+        // the correlation buffer (c N^2 doubles) holds the per-workgroup maxima
+        launch_xcorr_shift_f64(j->plan, j->ref_spec.get(), j->sspec.get(), c,
+            reinterpret_cast<RowBest*>(j->corr.get()), j->swork.get(), sh + 2 * o, s);
         launch_saa_add(f, p.frame_dtype, c, X, Y, sh + 2 * o, j->sum.get(), s);
     }
 }

@@ -366,6 +366,105 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict
     }
 }
 
+// k_rows_c2r with calculate_displacement's search (include/crosscorrel.h:166-179)
+// fused in: instead of writing its correlation rows, a workgroup writes the
+// first maximum of its rows in row-major order to best[f][blockIdx.x], and
+// k_best_to_shift merges those in order.  The correlation never reaches memory.
+__device__ inline void better_of(double& bv, int64_t& bi, double v, int64_t i)
+{
+    // (v, i) replaces (bv, bi) when it is larger, or equal with a lower index
+    if (bv < v || (!(v < bv) && i < bi)) {
+        bv = v;
+        bi = i;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __restrict__ G, int X, int Y, int lx,
+    int ppw, int hx, const double2* __restrict__ tw, RowBest* __restrict__ best)
+{
+    extern __shared__ char smem[];
+    __shared__ double wv[kThreads / 64];
+    __shared__ int64_t wi[kThreads / 64];
+    double2* A = reinterpret_cast<double2*>(smem);
+    double2* B = A + ppw * X;
+    const int64_t f = blockIdx.y;
+    const int pair0 = blockIdx.x * ppw;
+    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
+        const int p = t >> lx, k = t & (X - 1);
+        const int64_t y0 = 2 * (pair0 + p);
+        const double2* g0 = G + (f * Y + y0) * hx;
+        const double2* g1 = g0 + hx;
+        double2 h0, h1;
+        if (k <= X / 2) {
+            h0 = g0[k];
+            h1 = g1[k];
+            if (k == 0 || k == X / 2) {
+                h0.y = 0.0;
+                h1.y = 0.0;
+            }
+        } else {
+            h0 = cconj(g0[X - k]);
+            h1 = cconj(g1[X - k]);
+        }
+        A[t] = cadd(h0, cmul_pi(h1));
+    }
+    __syncthreads();
+    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
+    // the scan starts from element 0 (max_element's first candidate) in the
+    // workgroup that holds it, from nothing elsewhere
+    double bv = -INFINITY;
+    int64_t bi = INT64_MAX;
+    if (blockIdx.x == 0) {
+        bv = Z[0].x;
+        bi = 0;
+    }
+    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
+        const int p = t >> lx, x = t & (X - 1);
+        const int64_t i0 = (2 * (int64_t)(pair0 + p)) * X + x; // Z.x is row y0, Z.y row y0 + 1
+        const double2 z = Z[t];
+        better_of(bv, bi, z.x, i0);
+        better_of(bv, bi, z.y, i0 + X);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o);
+        const int64_t oi = __shfl_xor(bi, o);
+        better_of(bv, bi, ov, oi);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wv[threadIdx.x >> 6] = bv;
+        wi[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+            better_of(bv, bi, wv[k], wi[k]);
+        best[f * gridDim.x + blockIdx.x] = RowBest { bv, bi };
+    }
+}
+
+// merges the workgroups' maxima of each frame in index order and applies the
+// signed wrap x >= (n+1)/2 -> x - n
+__global__ void k_best_to_shift(const RowBest* __restrict__ best, int nblk, int64_t nf, int X, int Y,
+    int32_t* __restrict__ shifts)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const RowBest* b = best + f * nblk;
+    double bv = b[0].v;
+    int64_t bi = b[0].i;
+    for (int k = 1; k < nblk; ++k)
+        better_of(bv, bi, b[k].v, b[k].i);
+    int32_t sx = (int32_t)(bi % X), sy = (int32_t)(bi / X);
+    if (!(sx < (X + 1) / 2))
+        sx -= X;
+    if (!(sy < (Y + 1) / 2))
+        sy -= Y;
+    shifts[2 * f] = sx;
+    shifts[2 * f + 1] = sy;
+}
+
 // Complex 2-D transform passes (in place), used by the synthesis.
 __global__ __launch_bounds__(kThreads) void k_rows_c2c(double2* __restrict__ data, int X, int lx, int rpw,
     const double2* __restrict__ tw, int inv)
@@ -496,6 +595,23 @@ void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf
     HalfPlane hp {};
     launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
     launch_cols_fwd<double2, 1, double2>(p, work, nf, out, hp, s);
+}
+
+void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
+    RowBest* best, double2* work, int32_t* shifts, hipStream_t s)
+{
+    const int X = p->X, Y = p->Y, hx = X / 2 + 1;
+    const int cb = per_wg<double2>(Y, 64);
+    hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
+        2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+    SMIP_HIP(hipGetLastError());
+    const int ppw = per_wg<double2>(X, Y / 2);
+    const int nblk = (Y / 2) / ppw;
+    hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
+        s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
+    SMIP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_best_to_shift, dim3((unsigned)((nf + 63) / 64)), dim3(64), 0, s, best, nblk, nf, X, Y, shifts);
+    SMIP_HIP(hipGetLastError());
 }
 
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf, double* corr,

@@ -81,6 +81,23 @@ void launch_argmax_shift(const double* corr, int64_t nf, int32_t ncols, int32_t 
 }
 
 template <typename In>
+__device__ inline double shifted_abs(const In* __restrict__ frames, int64_t f, int32_t X, int32_t Y, int32_t col,
+    int32_t row, const int32_t* __restrict__ shifts)
+{
+    // distance = -shift; shifted(d)(col, row) = in(col - d0, row - d1) on the
+    // clamped window, zero elsewhere
+    const int32_t dx = -shifts[2 * f], dy = -shifts[2 * f + 1];
+    const int32_t sx = min(max(dx, 0), X - 1), ex = min(max(dx + X, 1), X);
+    const int32_t sy = min(max(dy, 0), Y - 1), ey = min(max(dy + Y, 1), Y);
+    double v = 0.0;
+    if (col >= sx && col < ex && row >= sy && row < ey) {
+        const double xd = (double)frames[f * (int64_t)X * Y + (int64_t)(row - dy) * X + (col - dx)];
+        v = xd < 0 ? -xd : xd;
+    }
+    return v;
+}
+
+template <typename In>
 __global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
     const int32_t* __restrict__ shifts, double* __restrict__ sum)
 {
@@ -89,20 +106,19 @@ __global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, 
         return;
     const int32_t col = (int32_t)(p % X), row = (int32_t)(p / X);
     double acc = sum[p];
-    for (int64_t f = 0; f < nf; ++f) {
-        // distance = -shift; shifted(d)(col, row) = in(col - d0, row - d1) on the
-        // clamped window, zero elsewhere
-        const int32_t dx = -shifts[2 * f], dy = -shifts[2 * f + 1];
-        const int32_t sx = min(max(dx, 0), X - 1), ex = min(max(dx + X, 1), X);
-        const int32_t sy = min(max(dy, 0), Y - 1), ey = min(max(dy + Y, 1), Y);
-        double v = 0.0;
-        if (col >= sx && col < ex && row >= sy && row < ey) {
-            const In x = frames[f * (int64_t)X * Y + (int64_t)(row - dy) * X + (col - dx)];
-            const double xd = (double)x;
-            v = xd < 0 ? -xd : xd;
-        }
-        acc += v;
+    int64_t f = 0;
+    // eight frames' loads in flight at once, added in frame order
+    for (; f + 8 <= nf; f += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            v[k] = shifted_abs(frames, f + k, X, Y, col, row, shifts);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            acc += v[k];
     }
+    for (; f < nf; ++f)
+        acc += shifted_abs(frames, f, X, Y, col, row, shifts);
     sum[p] = acc;
 }
 

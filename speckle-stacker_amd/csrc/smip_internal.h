@@ -212,6 +212,15 @@ void launch_forward_f64(const FftPlan* p, const void* frames, int dtype, int64_t
 void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
     double2* out, double2* work, hipStream_t s);
 // c = c2r(conj(R_ref) * R_f) for nf frames; out nf * nrows * ncols doubles.
+// Per-workgroup first maximum of a correlation (value, row-major index).
+struct RowBest {
+    double v;
+    int64_t i;
+};
+// The cross-correlation of each spectrum with the reference, reduced straight to
+// its displacement: shifts[2f], shifts[2f+1].  `best` holds nf * nrows / 2 entries.
+void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
+    RowBest* best, double2* work, int32_t* shifts, hipStream_t s);
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
     double* corr, double2* work, hipStream_t s);
 // unnormalised backward complex 2-D FFT, FP64, in place on nrows x ncols.
