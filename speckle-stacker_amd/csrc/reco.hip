@@ -507,31 +507,46 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
             fix_stage_chunk(reinterpret_cast<int4*>(buf[(c + 1) & 1]), src4, info, c + 1, nsteps,
                 ctotal[c + 1] * kParts, cap, lane);
         const int q0 = c * kChunk, q1 = min(nsteps, q0 + kChunk);
-        PendingRec nr {};
-        if (staged) {
-            const StepInfo i0 = info[q0];
-            if (lane < i0.cnt)
-                nr = cb[i0.off + lane];
+        // The chunk's step scalars, read once: the visit loop below is unrolled, so
+        // each visit's stay in registers of their own, and a visit's only LDS round
+        // trip is the one that must follow its predecessor's write (partner phases,
+        // the revisit check, and the next visit's records behind them).
+        int32_t n0[kChunk], fp[kChunk], cn[kChunk], of[kChunk], er[kChunk], rv[kChunk];
+        double2 Sv[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const bool in = q0 + k < q1;
+            const StepInfo si = info[in ? q0 + k : q0];
+            n0[k] = in ? __builtin_amdgcn_readfirstlane(si.n0) : -1;
+            fp[k] = __builtin_amdgcn_readfirstlane(si.fp);
+            cn[k] = in ? __builtin_amdgcn_readfirstlane(si.cnt) : 0;
+            of[k] = __builtin_amdgcn_readfirstlane(si.off);
+            er[k] = __builtin_amdgcn_readfirstlane(si.err);
+            rv[k] = __builtin_amdgcn_readfirstlane(si.rev);
+            Sv[k] = si.S;
         }
-        for (int j = q0; j < q1; ++j) {
-            const StepInfo si = info[j];
+        PendingRec nr {};
+        if (staged && lane < cn[0])
+            nr = cb[of[0] + lane];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int j = q0 + k;
             const PendingRec r0 = nr; // this visit's first 64 records, prefetched from LDS
-            // partner state of r0 first, then the next visit's prefetch behind it
             double2 pu0 = mk<double2>(kNotYet, 0.0), pv0 = mk<double2>(kNotYet, 0.0);
-            if (staged && lane < si.cnt) {
+            if (staged && lane < cn[k]) {
                 pu0 = ppos[max(r0.ju, 0)];
                 pv0 = ppos[max(r0.jv, 0)];
             }
-            if (staged && j + 1 < q1) {
-                const StepInfo sn = info[j + 1];
+            const double fpx = ppos[fp[k]].x; // flagged by an earlier visit of this window?
+            if (k + 1 < kChunk) {
                 nr = PendingRec {};
-                if (lane < sn.cnt)
-                    nr = cb[sn.off + lane];
+                if (staged && lane < cn[k + 1])
+                    nr = cb[of[k + 1] + lane];
             }
-            if (si.n0 < 0)
+            if (n0[k] < 0)
                 continue;
-            if (si.rev && ppos[si.fp].x != kNotYet)
-                continue; // flagged by an earlier visit of this window
+            if (rv[k] && fpx != kNotYet)
+                continue;
             ++calls;
             double sr = 0.0, sim = 0.0;
             int ne = 0; // count + (error << 16)
@@ -558,27 +573,27 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
             };
             auto take_at = [&](const PendingRec& r) { take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)]); };
             if (staged) {
-                if (lane < si.cnt)
+                if (lane < cn[k])
                     take(r0, pu0, pv0);
-                for (int t = 64 + lane; t < si.cnt; t += 64) // rare: more than 64 records
-                    take_at(cb[si.off + t]);
+                for (int t = 64 + lane; t < cn[k]; t += 64) // rare: more than 64 records
+                    take_at(cb[of[k] + t]);
             } else {
-                for (int t = lane; t < si.cnt; t += 64)
+                for (int t = lane; t < cn[k]; t += 64)
                     take_at(rec[(int64_t)j * cap + t]);
             }
             wave_sum3(ne, sr, sim);
-            if ((ne >> 16) || si.err) {
+            if ((ne >> 16) || er[k]) {
                 if (lane == 0)
                     status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
                 aborted = true;
                 break;
             }
-            const int tn = si.n0 + (ne & 0xffff);
+            const int tn = n0[k] + (ne & 0xffff);
             if (tn > 0) {
                 if (lane == 0) {
                     double ap;
-                    ppos[si.fp] = point_phase(si.S.x + sr, si.S.y + sim, tn, &ap);
-                    pcons[si.fp] = ap;
+                    ppos[fp[k]] = point_phase(Sv[k].x + sr, Sv[k].y + sim, tn, &ap);
+                    pcons[fp[k]] = ap;
                 }
                 ++flagged;
                 wave_lds_order();
