@@ -650,6 +650,25 @@ def main() -> int:
     streamed = streamed_leg(ctx, frames, p)
     log(f"streamed from host: {streamed}")
 
+    # one GPU's share of the stack split frame-wise over 8 GPUs: its partial
+    # bispectrum (the full FP64 cell array) is what one reduce step would send
+    share = max(1, n // 8)
+    js = S.Job(ctx, p)
+    js.add_frames(dev, 0, share)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    js.add_frames(dev, share, share)
+    ctx.synchronize()
+    share_ms = (time.perf_counter() - t0) * 1e3
+    js.close()
+    sizes = [2 * (d // 2) + 1 for d in (N, N, b, b)]
+    base = (sizes[0] - sizes[0] // 2) * sizes[1] * (sizes[2] - sizes[2] // 2) * sizes[3]
+    gpu_share = {"frames": share, "accumulate_ms": round(share_ms, 2),
+                 "partial_bispectrum_bytes": 16 * base,
+                 "note": "frames of one GPU when the stack is split over 8; the partial FP64 bispectrum "
+                         "is reduced to the GPU that recovers the phases"}
+    log(f"one GPU's share of 8: {gpu_share}")
+
     out = {
         "metric": "bispectrum frames/s @256^2,b=48,p=128 vs plain FP64 GPU path; p50/p99 job latency",
         "value": round(fps, 1),
@@ -676,6 +695,7 @@ def main() -> int:
                      "kernel": "k_accum_tile", "flops_per_launch": flops_per_launch,
                      "avg_launch_ms": round(tile_ms / launches, 4)},
         "streamed_from_host": streamed,
+        "gpu_share_of_8": gpu_share,
     }
 
     # ---- complete jobs: latency and accuracy ---------------------------------
