@@ -412,13 +412,18 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
     cv = threading.Condition()
     groups = []
 
+    # longest first: the batch's last sequences to start are its shortest, so the
+    # makespan does not end on one large sequence started late (the triple products
+    # grow about as N^2 b^2 per frame)
+    order = sorted(range(nseq), key=lambda k: -seqs[k]["n"] * (seqs[k]["N"] * seqs[k]["b"]) ** 2)
+
     def worker():
         while True:
             with lock:
-                i = nxt[0]
+                if nxt[0] >= nseq:
+                    return
+                i = order[nxt[0]]
                 nxt[0] += 1
-            if i >= nseq:
-                return
             try:
                 q = seqs[i]
                 dev = pools[(q["N"], q["kind"])]
@@ -491,7 +496,8 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
                               f"shift-and-add timed on the first {plain_sample} frames of each sequence and scaled "
                               "to its frame count, plus its phase recovery and synthesis",
             "mix": "side {64,128,256,512}, frames log-uniform 40-5000, b U[16,64], p=2b, "
-                   "binary/u8, triple/u16, extended/f32; frames cycle through 64 per side and type"}
+                   "binary/u8, triple/u16, extended/f32; frames cycle through 64 per side and type",
+            "order": "longest first (frames x N^2 b^2)"}
 
 
 def precision_leg(ctx, n: int) -> dict:
