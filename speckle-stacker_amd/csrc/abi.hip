@@ -1101,7 +1101,7 @@ JobSizes job_sizes(const smip_job_params& p)
     if (mixed && z.vy_tile == 0 && tuned)
         z.vy_tile = tuned_vy;
     if (mixed && z.vy_tile == 0) {
-        z.vy_tile = tile_default_vy(z.L.max4 - z.L.min4 + 1); // as tile_plan_create
+        z.vy_tile = tile_default_vy(z.L, (int)p.ncols, (int)p.nrows); // as tile_plan_create
     }
     const int pad = mixed ? tile_plan_pad_needed(z.d, z.vy_tile) : 0;
     const int tail = mixed ? tile_plan_tail_needed(z.d, (int)p.nrows, z.vy_tile) : 0;

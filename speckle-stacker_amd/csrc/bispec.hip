@@ -275,18 +275,29 @@ __global__ __launch_bounds__(64) void k_accum_tile(const int4* __restrict__ item
 
 static const int kVyTiles[] = { 9, 13, 17, 25, 33 };
 
-// Default vy tile for nvy valid vy values: least modelled time, chunks x tile
-// width x a per-width factor.  The factor is 1 except for 33, whose register
-// footprint halves occupancy: measured 1.65x the time per cell of 17 and 25 at
-// 256x256, b = 48.  Per-item overhead is about 0.2 cell and is ignored.  Ties go
-// to the wider tile.
-int tile_default_vy(int nvy)
+// Default vy tile: least modelled time, chunks x tile width / (the width's
+// throughput per computed cell x how fully the items fill the GPU).  Fitted to the
+// kernel's measured rates at 128x128, 256x256 and 512x512 for b = 16..62:
+// * per-cell throughput relative to 17 and 25: 0.9 for 9, 0.8 for 13, 0.7 for 33
+//   (whose register footprint halves occupancy);
+// * fill: items / 6144, at most 1 (smaller frames need narrow tiles to have
+//   enough work items).
+// Ties go to the wider tile.
+int tile_default_vy(const LoopBounds& L, int X, int Y)
 {
+    const int nvy = L.max4 - L.min4 + 1;
+    int64_t pairs = 0; // (ux, vx) columns of the reference loop
+    for (int cu = 0; cu <= (X >> 1); ++cu)
+        pairs += std::min(cu, -L.min3) + 1;
+    const int64_t nblk = (Y + 63) / 64;
     int best = kVyTiles[0];
     double best_cost = 1e30;
     for (int c : kVyTiles) {
-        const double cost = (double)((nvy + c - 1) / c) * c * (c == 33 ? 1.65 : 1.0);
-        if (cost < best_cost || (cost == best_cost && c > best)) {
+        const int64_t chunks = (nvy + c - 1) / c;
+        const double eff = c == 9 ? 0.9 : c == 13 ? 0.8 : c == 33 ? 0.7 : 1.0;
+        const double fill = std::min(1.0, (double)(pairs * nblk * chunks) / 6144.0);
+        const double cost = (double)(chunks * c) / (eff * fill);
+        if (cost < best_cost * (1.0 - 1e-9) || (cost <= best_cost * (1.0 + 1e-9) && c > best)) {
             best = c;
             best_cost = cost;
         }
@@ -301,7 +312,7 @@ TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& 
 {
     const int nvy = L.max4 - L.min4 + 1;
     if (vy_tile <= 0)
-        vy_tile = tile_default_vy(nvy);
+        vy_tile = tile_default_vy(L, hp.ncols, hp.nrows);
     bool ok = false;
     for (int c : kVyTiles)
         ok |= (c == vy_tile);

@@ -236,7 +236,7 @@ struct TilePlan; // work-item table of the batched accumulate
 TilePlan* tile_plan_create(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, int vy_tile, hipStream_t stream);
 void tile_plan_destroy(TilePlan*);
 int tile_plan_vy_tile(const TilePlan*);
-int tile_default_vy(int nvy); // vy tile of the batched kernel when none is given
+int tile_default_vy(const LoopBounds& L, int X, int Y); // vy tile of the batched kernel when none is given
 int tile_plan_pad_needed(const Desc& d, int vy_tile);
 int tile_plan_tail_needed(const Desc& d, int nrows, int vy_tile);
 void launch_accum_tile(const TilePlan* tp, const HalfPlane& hp, const float2* H, int64_t nf, double2* B64,
