@@ -1477,7 +1477,11 @@ extern "C" SMIP_API int smip_job_destroy(smip_job* j)
         if (!j)
             return;
         ensure_device(j->ctx);
-        (void)hipStreamSynchronize(j->stream);
+        // copies of pushed frames still pending on the copy stream read the pinned
+        // slots freed below
+        for (hipStream_t st : { j->ustream, j->fstream, j->sstream, j->stream })
+            if (st)
+                (void)hipStreamSynchronize(st);
         delete j;
     });
 }

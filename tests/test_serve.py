@@ -81,6 +81,35 @@ def test_graph_replays_match_eager_launches(ctx, monkeypatch, precision):
     _assert_identical(outs["graph"], outs["eager"])
 
 
+def test_host_batches_of_a_large_job_match_device_stack(ctx):
+    """A job large enough for its host batches to overlap their copies and FFT with
+    the previous batch's triple products (256x256, b=48, 128-frame batches): pushed
+    in uneven chunks, added in one host call, and from a device stack, the outputs
+    are identical."""
+    frames = _frames("triple256", 256, 5 * 128 + 37, seed_offset=3)
+    p = S.job_params(256, 256, 48, 128, frames.dtype, frames_per_batch=128)
+    ref = S.Job(ctx, p)
+    ref.add_frames(S.Frames(ctx, frames))
+    ref.finalize()
+    want = _outputs(ref)
+    ref.close()
+    pushed = S.Job(ctx, p)
+    for a, b in ((0, 1), (1, 150), (150, 151), (151, 400), (400, len(frames))):
+        pushed.push_frames_host(frames[a:b])
+    pushed.finalize()
+    _assert_identical(_outputs(pushed), want)
+    pushed.close()
+    added = S.Job(ctx, p)
+    added.add_frames_host(frames)
+    added.finalize()
+    _assert_identical(_outputs(added), want)
+    added.close()
+    # closing a job whose pushed frames are still being copied
+    left = S.Job(ctx, p)
+    left.push_frames_host(frames[:200])
+    left.close()
+
+
 def test_push_mixing_rules(ctx):
     frames = _frames("binary128", 64, 8)
     job = S.Job(ctx, S.job_params(64, 64, 16, 24, frames.dtype, frames_per_batch=32))
