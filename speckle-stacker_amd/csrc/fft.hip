@@ -12,6 +12,7 @@
 // several transforms so every stage keeps all 256 lanes busy.  Real frames are
 // transformed two rows at a time (z = row0 + i row1) and split afterwards.
 // Twiddles come from tables computed on the host in double precision.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -520,12 +521,24 @@ static int per_wg(int n, int cap)
     return p;
 }
 
+// Transforms per workgroup along contiguous rows: just enough that every radix-4
+// stage keeps all 256 lanes busy (n/4 butterflies per transform), so small LDS
+// footprints let several workgroups share a CU and hide each other's latency.
+// Column passes keep per_wg's wider blocks: their loads gather a row segment per
+// transform and need that width to stay coalesced.
+template <typename V>
+static int rows_per_wg(int n, int cap)
+{
+    const int fill = std::max(1, (4 * kThreads) / n);
+    return std::min(per_wg<V>(n, cap), fill);
+}
+
 template <typename V>
 static void launch_rows_fwd(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval, V* G,
     hipStream_t s)
 {
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
-    const int ppw = per_wg<V>(X, Y / 2);
+    const int ppw = rows_per_wg<V>(X, Y / 2);
     dim3 grid((Y / 2) / ppw, (unsigned)nf);
     const size_t lds = 2 * sizeof(V) * ppw * X;
     switch (dtype) {
@@ -605,7 +618,7 @@ void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const dou
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
         2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
-    const int ppw = per_wg<double2>(X, Y / 2);
+    const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;
     hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
         s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
@@ -622,7 +635,7 @@ void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* 
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
         2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
-    const int ppw = per_wg<double2>(X, Y / 2);
+    const int ppw = rows_per_wg<double2>(X, Y / 2);
     hipLaunchKernelGGL(k_rows_c2r, dim3((Y / 2) / ppw, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
         s, work, X, Y, p->lx, ppw, hx, p->twx64, corr);
     SMIP_HIP(hipGetLastError());
@@ -632,7 +645,7 @@ void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipS
 {
     (void)work;
     const int X = p->X, Y = p->Y;
-    const int rpw = per_wg<double2>(X, Y);
+    const int rpw = rows_per_wg<double2>(X, Y);
     hipLaunchKernelGGL(k_rows_c2c, dim3(Y / rpw), dim3(kThreads), 2 * sizeof(double2) * rpw * X, s, data, X, p->lx,
         rpw, p->twx64, 1);
     SMIP_HIP(hipGetLastError());
