@@ -196,6 +196,25 @@ def test_tile_shapes(ctx, vy_tile):
     assert np.array_equal(got.bispectrum == 0, B == 0)
 
 
+@pytest.mark.parametrize("N", [8, 2048])
+def test_transform_size_extremes(ctx, N):
+    """The smallest and largest frame sides the engine takes: every transform (the
+    FP32 forward of the bispectrum, the FP64 ones of shift-and-add and synthesis)
+    at lengths 8 and 2048, whose workgroup shapes and LDS budgets differ from the
+    usual sizes, through a whole job against the oracle."""
+    frames = _frames(3, N, seed=23)
+    b, p = 2, 4
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=S.PRECISION_MIXED)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    assert _rel(got.bispectrum, want.bispectrum) < 2e-5
+    assert _rel(got.powerspec, want.powerspec) < 1e-5
+    assert _rel_l2(got.image, want.image) < 1e-5
+
+
 def test_reco_windowed_matches_sequential(ctx, monkeypatch):
     """The windowed schedule against the one-visit-at-a-time kernel and the oracle:
     identical flags, phases equal up to the FP64 summation order."""
