@@ -181,6 +181,90 @@ __device__ V* lds_fft(V* a, V* b, int n, int log2n, int nb, const V* __restrict_
     return a;
 }
 
+// ---------------------------------------------------------------------------
+// In-place LDS FFT (decimation in frequency) of nb transforms of length
+// n = 2^log2n stored back to back in a: a radix-2 stage first when log2n is odd,
+// then radix-4 stages.  Every butterfly reads and writes the same four (two)
+// locations, so one buffer and one barrier per stage suffice -- half the LDS of
+// the Stockham form, which lets a column pass hold twice the columns.  The
+// result is in digit-reversed order: X[k] is at dif_pos(k).
+//   radix-4 stage of span L (quarter Q = L/4), butterfly j in [0, Q) of group g:
+//     x_r = a[g L + j + r Q]; DFT-4 of (x_0..x_3); output r scaled by w_L^(r j)
+//   radix-2 stage: x_0 + x_1, (x_0 - x_1) w_n^j
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict__ tw, bool inv)
+{
+    const int nthreads = blockDim.x;
+    const int tid = threadIdx.x;
+    int L = n, lL = log2n;
+    if (log2n & 1) {
+        const int h = n >> 1;
+        for (int t = tid; t < nb * h; t += nthreads) {
+            const int f = t >> (log2n - 1), j = t & (h - 1);
+            V* x = a + f * n + j;
+            const V x0 = x[0], x1 = x[h];
+            V w = tw[j];
+            if (inv)
+                w = cconj(w);
+            x[0] = cadd(x0, x1);
+            x[h] = cmul(csub(x0, x1), w);
+        }
+        __syncthreads();
+        L = h;
+        lL = log2n - 1;
+    }
+    while (L >= 4) {
+        const int Q = L >> 2, lQ = lL - 2;
+        const int step = n >> lL; // w_L = tw[step]
+        for (int t = tid; t < nb * (n >> 2); t += nthreads) {
+            const int f = t >> (log2n - 2), b = t & ((n >> 2) - 1);
+            const int g = b >> lQ, j = b & (Q - 1);
+            V* x = a + f * n + g * L + j;
+            const V x0 = x[0], x1 = x[Q], x2 = x[2 * Q], x3 = x[3 * Q];
+            const V a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+            const V a3 = inv ? cmul_pi(csub(x1, x3)) : cmul_mi(csub(x1, x3));
+            V y1 = cadd(a1, a3), y2 = csub(a0, a2), y3 = csub(a1, a3);
+            if (j > 0) {
+                V w1 = tw[step * j], w2 = tw[2 * step * j], w3 = tw[3 * step * j];
+                if (inv) {
+                    w1 = cconj(w1);
+                    w2 = cconj(w2);
+                    w3 = cconj(w3);
+                }
+                y1 = cmul(y1, w1);
+                y2 = cmul(y2, w2);
+                y3 = cmul(y3, w3);
+            }
+            x[0] = cadd(a0, a2);
+            x[Q] = y1;
+            x[2 * Q] = y2;
+            x[3 * Q] = y3;
+        }
+        __syncthreads();
+        L = Q;
+        lL = lQ;
+    }
+}
+
+// position of X[k] after lds_fft_dif: the radix-2 split (odd log2n) selects the
+// half by k's lowest bit, then the base-4 digits of the rest reverse
+__device__ inline int dif_pos(int k, int log2n)
+{
+    int half = 0, m = log2n;
+    if (log2n & 1) {
+        half = (k & 1) << (log2n - 1);
+        k >>= 1;
+        m = log2n - 1;
+    }
+    int r = 0;
+    for (int d = 0; d < m; d += 2) {
+        r = (r << 2) | (k & 3);
+        k >>= 2;
+    }
+    return half | r;
+}
+
 template <typename S, typename In>
 __device__ inline S load_sample(const In* p, int64_t i, bool absval)
 {
@@ -190,6 +274,10 @@ __device__ inline S load_sample(const In* p, int64_t i, bool absval)
 
 // LDS budget per workgroup (two ping-pong buffers): 64 KiB.
 constexpr int kLdsBytes = 65536;
+// LDS of a column-pass workgroup.  Measured against 64 KiB (and against the
+// two-buffer Stockham form at 32 KiB): the shift-and-add at 512x512 is 8% faster,
+// at 128x128 13% slower, elsewhere within 2%.
+constexpr int kColLds = 32768;
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------------------
@@ -237,7 +325,6 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
 {
     extern __shared__ char smem[];
     V* A = reinterpret_cast<V*>(smem);
-    V* B = A + cb * Y;
     const int64_t f = blockIdx.y;
     const int kx0 = blockIdx.x * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -246,14 +333,15 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
         A[c * Y + y] = kx < hx ? G[(f * Y + y) * hx + kx] : mk<V>(0, 0);
     }
     __syncthreads();
-    const V* R = lds_fft<V>(A, B, Y, ly, cb, tw, false);
+    lds_fft_dif<V>(A, Y, ly, cb, tw, false);
+    const V* R = A; // digit-reversed: X[k] at dif_pos(k)
     if (MODE == 0) {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
             const int c = t / Y, r = t - c * Y;
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
-            const V v = R[c * Y + ((Y / 2 - r) & (Y - 1))];
+            const V v = R[c * Y + dif_pos((Y / 2 - r) & (Y - 1), ly)];
             Out o;
             o.x = v.x;
             o.y = -v.y;
@@ -265,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
-            const V v = R[c * Y + ky];
+            const V v = R[c * Y + dif_pos(ky, ly)];
             Out o;
             o.x = v.x;
             o.y = v.y;
@@ -302,7 +390,6 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
 {
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
-    double2* B = A + cb * Y;
     const int64_t f = blockIdx.y;
     const int kx0 = blockIdx.x * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -318,12 +405,12 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
         A[c * Y + y] = v;
     }
     __syncthreads();
-    const double2* Rs = lds_fft<double2>(A, B, Y, ly, cb, tw, true);
+    lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         const int kx = kx0 + c;
         if (kx < hx)
-            G[(f * Y + y) * hx + kx] = Rs[c * Y + y];
+            G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
     }
 }
 
@@ -487,18 +574,17 @@ __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ dat
 {
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
-    double2* B = A + cb * Y;
     const int x0 = blockIdx.x * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         A[c * Y + y] = (x0 + c < X) ? data[(int64_t)y * X + x0 + c] : mk<double2>(0, 0);
     }
     __syncthreads();
-    const double2* Z = lds_fft<double2>(A, B, Y, ly, cb, tw, inv != 0);
+    lds_fft_dif<double2>(A, Y, ly, cb, tw, inv != 0);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         if (x0 + c < X)
-            data[(int64_t)y * X + x0 + c] = Z[c * Y + y];
+            data[(int64_t)y * X + x0 + c] = A[c * Y + dif_pos(y, ly)];
     }
 }
 
@@ -521,14 +607,19 @@ static int per_wg(int n, int cap, int budget = kLdsBytes)
     return p;
 }
 
-// Columns per workgroup of a column pass: from a 32 KiB budget, so that several
-// workgroups share a CU (measured 10-13% faster than 64 KiB blocks at 256x256 and
-// 512x512; wider blocks with fully used 128 B row segments were slower).
+// Columns per workgroup of a column pass (one LDS buffer: the in-place FFT):
+// as many as kColLds holds, at most 64.
 template <typename V>
 static int cols_per_wg(int n)
 {
-    const int half = kLdsBytes / 2;
-    return half / (2 * (int)sizeof(V)) >= n ? per_wg<V>(n, 64, half) : per_wg<V>(n, 64); // 2048-point FP64: 64 KiB
+    int k = kColLds / ((int)sizeof(V) * n);
+    if (k < 1)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "transform length exceeds the LDS budget");
+    k = std::min(k, 64);
+    int p = 1;
+    while (p * 2 <= k)
+        p *= 2;
+    return p;
 }
 
 // Transforms per workgroup along contiguous rows: just enough that every radix-4
@@ -578,7 +669,7 @@ static void launch_cols_fwd(const FftPlan* p, const V* G, int64_t nf, Out* out, 
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<V>(Y);
     dim3 grid((hx + cb - 1) / cb, (unsigned)nf);
-    const size_t lds = 2 * sizeof(V) * cb * Y;
+    const size_t lds = sizeof(V) * cb * Y;
     hipLaunchKernelGGL((k_cols_fwd<V, MODE, Out>), grid, dim3(kThreads), lds, s, G, X, Y, p->ly, hx, cb, tw_y<V>(p),
         out, hp);
     SMIP_HIP(hipGetLastError());
@@ -624,7 +715,7 @@ void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const dou
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<double2>(Y);
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+        sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;
@@ -641,7 +732,7 @@ void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* 
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<double2>(Y);
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        2 * sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+        sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     hipLaunchKernelGGL(k_rows_c2r, dim3((Y / 2) / ppw, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
@@ -658,7 +749,7 @@ void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipS
         rpw, p->twx64, 1);
     SMIP_HIP(hipGetLastError());
     const int cb = cols_per_wg<double2>(Y);
-    hipLaunchKernelGGL(k_cols_c2c, dim3((X + cb - 1) / cb), dim3(kThreads), 2 * sizeof(double2) * cb * Y, s, data, X,
+    hipLaunchKernelGGL(k_cols_c2c, dim3((X + cb - 1) / cb), dim3(kThreads), sizeof(double2) * cb * Y, s, data, X,
         Y, p->ly, cb, p->twy64, 1);
     SMIP_HIP(hipGetLastError());
 }
