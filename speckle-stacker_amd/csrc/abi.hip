@@ -16,6 +16,7 @@
 #include <mutex>
 #include <map>
 #include <string>
+#include <unordered_map>
 #include <thread>
 #include <vector>
 
@@ -44,14 +45,112 @@ std::recursive_mutex& capture_guard()
     return m;
 }
 
+// Block cache.  hipFree and hipHostFree wait until every stream of the device is
+// idle, so a job released while other jobs run would stall them all (and every
+// thread queued behind the allocation lock).  Blocks that their owner has shown
+// to be idle (all streams that used them synchronised: IdleRelease scope) are
+// kept here instead, per device, and handed out again for requests of the same
+// size up to 1/8 smaller.  Cached blocks count as free memory; an allocation that
+// fails returns the device's cached blocks to HIP and tries again.
+namespace {
+thread_local int g_idle_release = 0;
+
+struct BlockCache {
+    std::mutex mu;
+    std::unordered_map<void*, std::pair<int, size_t>> dev_live; // every dev_malloc block: device, bytes
+    std::multimap<std::pair<int, size_t>, void*> dev_free;     // idle blocks by (device, bytes)
+    std::map<int, size_t> dev_cached, dev_cap;
+    std::unordered_map<void*, size_t> host_live;
+    std::multimap<size_t, void*> host_free;
+    size_t host_cached = 0;
+    static constexpr size_t kHostCap = size_t(4) << 30;
+
+    static BlockCache& get()
+    {
+        static auto* c = new BlockCache; // never destroyed: nothing is freed at exit
+        return *c;
+    }
+    template <typename M, typename K>
+    static void* take(M& m, const K& lo, const K& hi)
+    {
+        auto it = m.lower_bound(lo);
+        if (it == m.end() || hi < it->first)
+            return nullptr;
+        void* p = it->second;
+        m.erase(it);
+        return p;
+    }
+};
+
+size_t device_cap(BlockCache& c, int dev)
+{
+    auto it = c.dev_cap.find(dev);
+    if (it != c.dev_cap.end())
+        return it->second;
+    size_t total = 0;
+    if (hipDeviceTotalMem(&total, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        total = 0;
+    }
+    return c.dev_cap[dev] = total / 8;
+}
+
+int current_device()
+{
+    int dev = 0;
+    SMIP_HIP(hipGetDevice(&dev));
+    return dev;
+}
+} // namespace
+
+IdleRelease::IdleRelease() { ++g_idle_release; }
+IdleRelease::~IdleRelease() { --g_idle_release; }
+
+void cache_trim(int dev)
+{
+    BlockCache& c = BlockCache::get();
+    std::vector<void*> v;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (auto it = c.dev_free.lower_bound({ dev, 0 }); it != c.dev_free.end() && it->first.first == dev;) {
+            v.push_back(it->second);
+            c.dev_live.erase(it->second);
+            it = c.dev_free.erase(it);
+        }
+        c.dev_cached[dev] = 0;
+    }
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    for (void* p : v)
+        (void)hipFree(p);
+}
+
 void* dev_malloc(size_t bytes)
 {
-    std::lock_guard<std::recursive_mutex> lk(capture_guard());
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        fail(SMIP_ERR_NO_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    BlockCache& c = BlockCache::get();
+    const int dev = current_device();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (void* p = BlockCache::take(c.dev_free, std::make_pair(dev, bytes), std::make_pair(dev, bytes + bytes / 8))) {
+            c.dev_cached[dev] -= c.dev_live[p].second;
+            return p;
+        }
     }
+    void* p = nullptr;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        {
+            std::lock_guard<std::recursive_mutex> lk(capture_guard());
+            if (hipMalloc(&p, std::max<size_t>(bytes, 1)) == hipSuccess)
+                break;
+            (void)hipGetLastError();
+            p = nullptr;
+        }
+        if (attempt == 0)
+            cache_trim(dev);
+    }
+    if (!p)
+        fail(SMIP_ERR_NO_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.dev_live[p] = { dev, std::max<size_t>(bytes, 1) };
     return p;
 }
 
@@ -59,18 +158,51 @@ void dev_free(void* p)
 {
     if (!p)
         return;
+    BlockCache& c = BlockCache::get();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.dev_live.find(p);
+        if (it != c.dev_live.end()) {
+            const int dev = it->second.first;
+            const size_t n = it->second.second;
+            if (g_idle_release > 0 && c.dev_cached[dev] + n <= device_cap(c, dev)) {
+                c.dev_free.emplace(std::make_pair(dev, n), p);
+                c.dev_cached[dev] += n;
+                return;
+            }
+            c.dev_live.erase(it);
+        }
+    }
     std::lock_guard<std::recursive_mutex> lk(capture_guard());
     (void)hipFree(p);
 }
 
+size_t cache_device_bytes(int dev)
+{
+    BlockCache& c = BlockCache::get();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.dev_cached.find(dev);
+    return it == c.dev_cached.end() ? 0 : it->second;
+}
+
 void* host_malloc(size_t bytes)
 {
+    BlockCache& c = BlockCache::get();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (void* p = BlockCache::take(c.host_free, bytes, bytes + bytes / 8)) {
+            c.host_cached -= c.host_live[p];
+            return p;
+        }
+    }
     std::lock_guard<std::recursive_mutex> lk(capture_guard());
     void* p = nullptr;
     if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         fail(SMIP_ERR_NO_MEMORY, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
     }
+    std::lock_guard<std::mutex> lk2(c.mu);
+    c.host_live[p] = bytes;
     return p;
 }
 
@@ -78,14 +210,37 @@ void host_free(void* p)
 {
     if (!p)
         return;
+    BlockCache& c = BlockCache::get();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.host_live.find(p);
+        if (it != c.host_live.end()) {
+            if (g_idle_release > 0 && c.host_cached + it->second <= BlockCache::kHostCap) {
+                c.host_free.emplace(it->second, p);
+                c.host_cached += it->second;
+                return;
+            }
+            c.host_live.erase(it);
+        }
+    }
     std::lock_guard<std::recursive_mutex> lk(capture_guard());
     (void)hipHostFree(p);
 }
 
+// free memory as HIP reports it plus the blocks cached for reuse
 hipError_t mem_get_info(size_t* free_b, size_t* total_b)
 {
-    std::lock_guard<std::recursive_mutex> lk(capture_guard());
-    return hipMemGetInfo(free_b, total_b);
+    hipError_t e;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        e = hipMemGetInfo(free_b, total_b);
+    }
+    if (e == hipSuccess) {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            *free_b += cache_device_bytes(dev);
+    }
+    return e;
 }
 
 hipStream_t make_stream()
@@ -94,6 +249,77 @@ hipStream_t make_stream()
     hipStream_t s = nullptr;
     SMIP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return s;
+}
+
+// Streams for phase recovery.  A recovery is a chain of hundreds of short
+// dependent launches.  HIP maps a process's streams onto a few hardware queues,
+// and on a queue shared with other jobs' streams their batches wait behind the
+// whole chain.  A stream created with a CU mask gets a hardware queue of its own;
+// the mask here names every CU.  Kept per device for reuse, at most 8 per device;
+// beyond that (or where such a stream cannot be made) the caller's stream is used.
+namespace {
+struct RecoStreamPool {
+    std::mutex mu;
+    std::map<int, std::vector<hipStream_t>> idle;
+    std::map<int, int> made;
+    static RecoStreamPool& get()
+    {
+        static auto* p = new RecoStreamPool; // never destroyed: streams live to the end of the process
+        return *p;
+    }
+};
+} // namespace
+
+RecoStreamLease::RecoStreamLease(int dev, hipStream_t fallback)
+    : dev_(dev)
+    , use_(fallback)
+{
+    RecoStreamPool& P = RecoStreamPool::get();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& v = P.idle[dev];
+    if (!v.empty()) {
+        own_ = v.back();
+        v.pop_back();
+    } else if (P.made[dev] < 8) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) {
+            const int ncu = prop.multiProcessorCount;
+            std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
+            if (ncu % 32)
+                mask.back() = (1u << (ncu % 32)) - 1u;
+            std::lock_guard<std::recursive_mutex> lk2(capture_guard());
+            hipStream_t st = nullptr;
+            if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess) {
+                own_ = st;
+                ++P.made[dev];
+            }
+        }
+        (void)hipGetLastError();
+    }
+    if (own_)
+        use_ = own_;
+}
+
+void reco_streams_release(int dev)
+{
+    RecoStreamPool& P = RecoStreamPool::get();
+    std::vector<hipStream_t> v;
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        v.swap(P.idle[dev]);
+        P.made[dev] -= (int)v.size();
+    }
+    for (hipStream_t st : v)
+        destroy_stream(st);
+}
+
+RecoStreamLease::~RecoStreamLease()
+{
+    if (!own_)
+        return;
+    RecoStreamPool& P = RecoStreamPool::get();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.idle[dev_].push_back(own_);
 }
 
 void destroy_stream(hipStream_t s)
@@ -254,6 +480,7 @@ struct smip_job {
     uint64_t nframes = 0;
     bool finalized = false;
     uint64_t reco_calls = 0;
+    RecoWork* reco = nullptr; // phase-recovery scratch, made with the job
     std::unique_ptr<smip_bispec> view;
     std::vector<EventPair> events;
     double ms[5] = { 0, 0, 0, 0, 0 };
@@ -281,6 +508,7 @@ struct smip_job {
                 (void)hipGraphExecDestroy(g_saa[k]);
             host_free(pin[k]);
         }
+        reco_work_destroy(reco);
         tile_plan_destroy(tile);
         fft_plan_destroy(plan);
         destroy_stream(stream);
@@ -310,10 +538,6 @@ size_t dtype_size(int dtype)
         fail(SMIP_ERR_INVALID_ARGUMENT, "unknown dtype");
     }
 }
-} // namespace smip_amd
-
-namespace {
-
 // NextRecoIndex, src/phasereco.cpp:17-36
 void next_reco_index(double& r, double& phi, int32_t& i, int32_t& j)
 {
@@ -341,7 +565,11 @@ std::vector<int32_t> make_visit_list(double reco_radius, int64_t X, int64_t Y)
     std::vector<int32_t> v;
     double r = 0.0, phi = 0.0;
     int32_t i = 0, j = 0;
-    while (r <= reco_radius) {
+    // r never decreases and |(i, j)| > r - 1.5, so once r exceeds the map's
+    // half-diagonal by 2 no later visit is in range
+    const double hd = std::hypot((double)std::max<int64_t>(-mn0, mx0), (double)std::max<int64_t>(-mn1, mx1));
+    const double stop = std::min(reco_radius, hd + 2.0);
+    while (r <= stop) {
         next_reco_index(r, phi, i, j);
         if (i >= mn0 && i <= mx0 && j >= mn1 && j <= mx1) {
             v.push_back(i);
@@ -371,6 +599,10 @@ std::shared_ptr<const std::vector<int32_t>> visit_list(double reco_radius, int64
     cache[key] = v;
     return v;
 }
+
+} // namespace smip_amd
+
+namespace {
 
 void check_reco_geometry(const Desc& d, uint64_t xsize, uint64_t ysize)
 {
@@ -487,6 +719,8 @@ extern "C" SMIP_API int smip_ctx_destroy(smip_ctx* ctx)
         if (!ctx)
             return;
         destroy_stream(ctx->stream);
+        reco_streams_release(ctx->device);
+        cache_trim(ctx->device);
         delete ctx;
     });
 }
@@ -851,10 +1085,13 @@ static void reconstruct_into(smip_ctx* ctx, hipStream_t s, const Desc& d, const 
 {
     check_reco_geometry(d, xs, ys);
     (void)ctx;
-    const auto v = visit_list(reco_radius, (int64_t)xs, (int64_t)ys);
     std::unique_ptr<RecoWork, void (*)(RecoWork*)> w(
-        reco_work_create(d, (int32_t)xs, (int32_t)ys, v->data(), (int64_t)(v->size() / 2), s), reco_work_destroy);
+        reco_work_create(d, reco_geom(reco_radius, (int32_t)xs, (int32_t)ys, s)), reco_work_destroy);
     const int c = run_reconstruct(w.get(), B, is_f32, ph, fl, co, s);
+    {
+        IdleRelease idle; // run_reconstruct synchronised s
+        w.reset();
+    }
     if (calls)
         *calls = (uint64_t)c;
 }
@@ -1119,6 +1356,9 @@ JobSizes job_sizes(const smip_job_params& p)
             + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
     b += 2 * (uint64_t)z.batch * np * dtype_size(p.frame_dtype);         // two host staging buffers
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
+    // phase-recovery scratch, and the geometry's visit list (shared between jobs of
+    // one geometry, counted for each)
+    b += reco_scratch_bytes(z.d, (int32_t)X, (int32_t)Y) + reco_geom_bytes_estimate(p.reco_radius, (int32_t)X, (int32_t)Y);
     z.bytes = b;
     return z;
 }
@@ -1377,7 +1617,10 @@ void grow_shifts(smip_job* j, uint64_t total)
     if (j->shifts_cap)
         SMIP_HIP(hipMemcpyAsync(n.get(), j->shifts.get(), sizeof(int32_t) * 2 * j->shifts_cap,
             hipMemcpyDeviceToDevice, j->stream));
-    SMIP_HIP(hipStreamSynchronize(j->stream));
+    for (hipStream_t st : { j->sstream, j->fstream, j->stream })
+        if (st)
+            SMIP_HIP(hipStreamSynchronize(st));
+    IdleRelease idle; // every stream of the job is synchronised: the old array is idle
     j->shifts = std::move(n);
     j->shifts_cap = cap;
 }
@@ -1459,6 +1702,7 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         j->flags = dmake<uint8_t>(np);
         j->cons = dmake<double>(np);
         j->scratch = dmake<double>(16);
+        j->reco = reco_work_create(z.d, reco_geom(p->reco_radius, (int32_t)X, (int32_t)Y, s));
         SMIP_HIP(hipStreamSynchronize(s));
         auto v = std::make_unique<smip_bispec>();
         v->ctx = ctx;
@@ -1482,6 +1726,7 @@ extern "C" SMIP_API int smip_job_destroy(smip_job* j)
         for (hipStream_t st : { j->ustream, j->fstream, j->sstream, j->stream })
             if (st)
                 (void)hipStreamSynchronize(st);
+        IdleRelease idle; // nothing queued can touch the job's memory now
         delete j;
     });
 }
@@ -1700,11 +1945,16 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
         // bispectrum /= complex(n, 0)
         launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, s);
         hipEvent_t e0 = mark(j);
-        uint64_t calls = 0;
-        reconstruct_into(j->ctx, s, j->d, j->B64.get(), 0, p.ncols, p.nrows, p.reco_radius, j->phases.get(),
-            j->flags.get(), j->cons.get(), &calls);
-        j->reco_calls = calls;
-        hipEvent_t e1 = mark(j);
+        check_reco_geometry(j->d, p.ncols, p.nrows);
+        RecoStreamLease lease(j->ctx->device, s);
+        const hipStream_t rs = lease.get();
+        if (rs != s)
+            SMIP_HIP(hipStreamWaitEvent(rs, e0, 0));
+        j->reco_calls = (uint64_t)run_reconstruct(j->reco, j->B64.get(), 0, j->phases.get(), j->flags.get(),
+            j->cons.get(), rs);
+        hipEvent_t e1 = mark_on(j, rs);
+        if (rs != s)
+            SMIP_HIP(hipStreamWaitEvent(s, e1, 0));
         record(j, 2, e0, e1);
         hipEvent_t e2 = mark(j);
         const double norm = (double)(j->nframes * p.ncols * p.nrows);
@@ -1758,14 +2008,13 @@ extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n
             return;
         }
         ensure_device(jobs[0]->ctx);
-        hipStream_t cs = jobs[0]->stream; // the group's stream
+        RecoStreamLease lease(jobs[0]->ctx->device, jobs[0]->stream);
+        hipStream_t cs = lease.get(); // the group's stream
         std::vector<hipEvent_t> ev;
         std::vector<RecoWork*> work;
         auto cleanup = [&] {
             for (hipEvent_t e : ev)
                 (void)hipEventDestroy(e);
-            for (RecoWork* w : work)
-                reco_work_destroy(w);
         };
         try {
             auto event = [&](bool timing) {
@@ -1791,9 +2040,7 @@ extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n
             std::vector<int32_t> calls(n, 0), st(n, 0);
             for (uint64_t k = 0; k < n; ++k) {
                 smip_job* j = jobs[k];
-                const auto v = visit_list(j->p.reco_radius, (int64_t)j->p.ncols, (int64_t)j->p.nrows);
-                work.push_back(reco_work_create(j->d, (int32_t)j->p.ncols, (int32_t)j->p.nrows, v->data(),
-                    (int64_t)(v->size() / 2), cs));
+                work.push_back(j->reco);
                 B[k] = j->B64.get();
                 ph[k] = j->phases.get();
                 fl[k] = j->flags.get();
@@ -1896,6 +2143,8 @@ extern "C" SMIP_API int smip_job_get_powerspec(smip_job* j, double* out)
             tmp.get(), j->stream);
         SMIP_HIP(hipMemcpyAsync(out, tmp.get(), sizeof(double) * np, hipMemcpyDeviceToHost, j->stream));
         SMIP_HIP(hipStreamSynchronize(j->stream));
+        IdleRelease idle;
+        tmp.reset();
     });
 }
 

@@ -27,6 +27,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "smip_complex.h"
@@ -58,13 +63,29 @@ struct __align__(16) PendingRec {
     double2 c;
 };
 
+// Device copy of one geometry's visit list: the in-range spiral visits and, per
+// visit, the window slot of the first visit of the same point in its window.
+// Read-only once built, so jobs of one geometry share it.
+struct RecoGeom {
+    int32_t X = 0, Y = 0;
+    int64_t nvisits = 0;
+    void* block = nullptr; // visits, then first_pos
+    const int2* visits = nullptr;
+    const int32_t* first_pos = nullptr;
+    uint64_t bytes = 0;
+    ~RecoGeom() { dev_free(block); }
+};
+
+// Per-recovery scratch, carved from one allocation.
 struct RecoWork {
     RecoArgs args;
     int64_t nwin = 0;
     int cap = 0; // records per visit
-    int2* visits = nullptr;
-    int32_t* first_pos = nullptr; // window slot of the first visit of the same point
-    int64_t* map = nullptr;       // point -> (window << 32) | slot of its first visit
+    std::shared_ptr<const RecoGeom> geom;
+    void* block = nullptr;
+    const int2* visits = nullptr;
+    const int32_t* first_pos = nullptr;
+    int64_t* map = nullptr; // point -> (window << 32) | slot of its first visit
     double2* S = nullptr;
     int32_t* N = nullptr; // -1: the visit does nothing
     int32_t* E = nullptr;
@@ -800,26 +821,11 @@ __global__ __launch_bounds__(kSeqThreads) void k_reco_sequential(RecoArgs a, con
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <typename T>
-static T* dalloc(size_t n)
-{
-    return static_cast<T*>(dev_malloc(sizeof(T) * std::max<size_t>(n, 1)));
-}
+static size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-uint64_t reco_work_bytes(const Desc& d, int32_t xsize, int32_t ysize, int64_t nvisits)
+static RecoArgs reco_args(const Desc& d, int32_t xsize, int32_t ysize, int64_t nvisits)
 {
-    const int ncand = (d.max_idx[2] - d.min_idx[2] + 1) * (d.max_idx[3] - d.min_idx[3] + 1);
-    const int cap = std::min(ncand, 2 * kWindow);
-    return (uint64_t)nvisits * (sizeof(int2) + sizeof(int32_t)) + (uint64_t)xsize * ysize * sizeof(int64_t)
-        + (uint64_t)kWindow * (sizeof(double2) + 4 * sizeof(int32_t))
-        + (uint64_t)kWindow * cap * sizeof(PendingRec);
-}
-
-RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits,
-    hipStream_t stream)
-{
-    auto* w = new RecoWork;
-    RecoArgs& a = w->args;
+    RecoArgs a;
     a.d = d;
     a.X = xsize;
     a.Y = ysize;
@@ -828,46 +834,150 @@ RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const in
     a.pmx0 = -xsize / 2 + xsize - 1;
     a.pmx1 = -ysize / 2 + ysize - 1;
     a.nvisits = nvisits;
+    return a;
+}
+
+static int reco_cap(const Desc& d)
+{
     const int ncand = (d.max_idx[2] - d.min_idx[2] + 1) * (d.max_idx[3] - d.min_idx[3] + 1);
-    if (ncand > 64 * kBulkThreads) {
-        delete w;
+    if (ncand > 64 * kBulkThreads)
         fail(SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth too large for phase recovery (b/2 <= 63)");
-    }
-    w->cap = std::min(ncand, 2 * kWindow);
-    w->nwin = (nvisits + kWindow - 1) / kWindow;
-    // window slot of the first visit of each visit's point within its window
+    return std::min(ncand, 2 * kWindow);
+}
+
+// Byte offsets of the scratch arrays inside one block.
+struct ScratchLayout {
+    size_t map, S, N, E, cnt, rec, status, total;
+};
+
+static ScratchLayout scratch_layout(int32_t xsize, int32_t ysize, int cap)
+{
+    ScratchLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += up256(bytes);
+        return at;
+    };
+    L.map = take(sizeof(int64_t) * (size_t)xsize * ysize);
+    L.S = take(sizeof(double2) * kWindow);
+    L.N = take(sizeof(int32_t) * kWindow);
+    L.E = take(sizeof(int32_t) * kWindow);
+    L.cnt = take(sizeof(int32_t) * kWindow);
+    L.rec = take(sizeof(PendingRec) * (size_t)kWindow * cap);
+    L.status = take(sizeof(int32_t) * 4);
+    L.total = o;
+    return L;
+}
+
+uint64_t reco_scratch_bytes(const Desc& d, int32_t xsize, int32_t ysize)
+{
+    return scratch_layout(xsize, ysize, reco_cap(d)).total;
+}
+
+// Window slot of the first visit of each visit's point within its window.
+static std::vector<int32_t> first_positions(int32_t xsize, int32_t ysize, const int32_t* visits, int64_t nvisits)
+{
     std::vector<int32_t> first(std::max<int64_t>(nvisits, 1));
-    {
-        std::vector<int32_t> seen((size_t)xsize * ysize, -1);
-        for (int64_t s = 0; s < nvisits; s += kWindow) {
-            const int64_t e = std::min<int64_t>(s + kWindow, nvisits);
-            for (int64_t k = s; k < e; ++k) {
-                int32_t x = visits_host[2 * k], y = visits_host[2 * k + 1];
-                const size_t p = (size_t)((y < 0 ? y + ysize : y) * xsize + (x < 0 ? x + xsize : x));
-                if (seen[p] < s) // not yet visited in this window
-                    seen[p] = (int32_t)k;
-                first[k] = seen[p] - (int32_t)s;
-            }
+    std::vector<int32_t> seen((size_t)xsize * ysize, -1);
+    for (int64_t s = 0; s < nvisits; s += kWindow) {
+        const int64_t e = std::min<int64_t>(s + kWindow, nvisits);
+        for (int64_t k = s; k < e; ++k) {
+            int32_t x = visits[2 * k], y = visits[2 * k + 1];
+            const size_t p = (size_t)((y < 0 ? y + ysize : y) * xsize + (x < 0 ? x + xsize : x));
+            if (seen[p] < s) // not yet visited in this window
+                seen[p] = (int32_t)k;
+            first[k] = seen[p] - (int32_t)s;
         }
     }
+    return first;
+}
+
+static std::shared_ptr<const RecoGeom> geom_upload(int32_t xsize, int32_t ysize, const int32_t* visits,
+    int64_t nvisits, hipStream_t s)
+{
+    const std::vector<int32_t> first = first_positions(xsize, ysize, visits, nvisits);
+    const size_t n = (size_t)std::max<int64_t>(nvisits, 1);
+    const size_t vb = up256(sizeof(int2) * n);
+    auto g = std::make_shared<RecoGeom>();
+    g->X = xsize;
+    g->Y = ysize;
+    g->nvisits = nvisits;
+    g->bytes = vb + sizeof(int32_t) * n;
+    g->block = dev_malloc(g->bytes);
+    g->visits = static_cast<const int2*>(g->block);
+    g->first_pos = reinterpret_cast<const int32_t*>(static_cast<const char*>(g->block) + vb);
+    if (nvisits) { // one upload of both arrays, laid out as on the device
+        std::vector<char> h(g->bytes, 0);
+        std::memcpy(h.data(), visits, sizeof(int2) * nvisits);
+        std::memcpy(h.data() + vb, first.data(), sizeof(int32_t) * nvisits);
+        copy_sync(g->block, h.data(), g->bytes, hipMemcpyHostToDevice, s);
+    }
+    return g;
+}
+
+// Visit list of reconstruct_phases(radius) on an X x Y map, on the current device.
+// Kept per (device, radius, X, Y) for the life of the process, up to 64
+// geometries and 1 GiB; a geometry dropped from the cache lives on in the
+// recoveries that hold it.
+std::shared_ptr<const RecoGeom> reco_geom(double reco_radius, int32_t xsize, int32_t ysize, hipStream_t s)
+{
+    int dev = 0;
+    SMIP_HIP(hipGetDevice(&dev));
+    using Key = std::tuple<int, double, int32_t, int32_t>;
+    static std::mutex mu;
+    static auto* cache = new std::map<Key, std::shared_ptr<const RecoGeom>>; // never destroyed: no frees at exit
+    static uint64_t cached_bytes = 0;
+    const Key key { dev, reco_radius, xsize, ysize };
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache->find(key);
+    if (it != cache->end())
+        return it->second;
+    const auto v = visit_list(reco_radius, xsize, ysize);
+    auto g = geom_upload(xsize, ysize, v->data(), (int64_t)(v->size() / 2), s);
+    if (cache->size() >= 64 || cached_bytes + g->bytes > (1ull << 30)) {
+        cache->clear();
+        cached_bytes = 0;
+    }
+    (*cache)[key] = g;
+    cached_bytes += g->bytes;
+    return g;
+}
+
+// Upper estimate of a geometry's device bytes: in-range visits number about
+// 1.22 pi r^2, clipped to 1.23 X Y where the disc leaves the map.
+uint64_t reco_geom_bytes_estimate(double reco_radius, int32_t xsize, int32_t ysize)
+{
+    const double r = std::max(reco_radius, 0.0) + 2.0;
+    const double nv = std::min(1.25 * M_PI * r * r, 1.25 * (double)xsize * ysize) + 1024.0;
+    return (uint64_t)(nv * (sizeof(int2) + sizeof(int32_t))) + 256;
+}
+
+RecoWork* reco_work_create(const Desc& d, std::shared_ptr<const RecoGeom> g)
+{
+    const int cap = reco_cap(d);
+    auto* w = new RecoWork;
+    w->args = reco_args(d, g->X, g->Y, g->nvisits);
+    w->cap = cap;
+    w->nwin = (g->nvisits + kWindow - 1) / kWindow;
+    w->visits = g->visits;
+    w->first_pos = g->first_pos;
+    w->geom = std::move(g);
+    const ScratchLayout L = scratch_layout(w->args.X, w->args.Y, cap);
     try {
-        w->visits = dalloc<int2>(nvisits);
-        w->first_pos = dalloc<int32_t>(nvisits);
-        w->map = dalloc<int64_t>((size_t)xsize * ysize);
-        w->S = dalloc<double2>(kWindow);
-        w->N = dalloc<int32_t>(kWindow);
-        w->E = dalloc<int32_t>(kWindow);
-        w->cnt = dalloc<int32_t>(kWindow);
-        w->rec = dalloc<PendingRec>((size_t)kWindow * w->cap);
-        w->status = dalloc<int32_t>(4);
-        if (nvisits) {
-            copy_sync(w->visits, visits_host, sizeof(int2) * nvisits, hipMemcpyHostToDevice, stream);
-            copy_sync(w->first_pos, first.data(), sizeof(int32_t) * nvisits, hipMemcpyHostToDevice, stream);
-        }
+        w->block = dev_malloc(L.total);
     } catch (...) {
-        reco_work_destroy(w);
+        delete w;
         throw;
     }
+    char* b = static_cast<char*>(w->block);
+    w->map = reinterpret_cast<int64_t*>(b + L.map);
+    w->S = reinterpret_cast<double2*>(b + L.S);
+    w->N = reinterpret_cast<int32_t*>(b + L.N);
+    w->E = reinterpret_cast<int32_t*>(b + L.E);
+    w->cnt = reinterpret_cast<int32_t*>(b + L.cnt);
+    w->rec = reinterpret_cast<PendingRec*>(b + L.rec);
+    w->status = reinterpret_cast<int32_t*>(b + L.status);
     return w;
 }
 
@@ -875,9 +985,7 @@ void reco_work_destroy(RecoWork* w)
 {
     if (!w)
         return;
-    for (void* p : { (void*)w->visits, (void*)w->first_pos, (void*)w->map, (void*)w->S, (void*)w->N, (void*)w->E,
-             (void*)w->cnt, (void*)w->rec, (void*)w->status })
-        dev_free(p);
+    dev_free(w->block);
     delete w;
 }
 
@@ -990,17 +1098,19 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
             hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(64), 0, s, dtab, win);
         }
         SMIP_HIP(hipGetLastError());
-        std::vector<int32_t> st(4);
+        std::vector<int32_t> st(4 * (size_t)n);
+        for (int k = 0; k < n; ++k)
+            SMIP_HIP(hipMemcpyAsync(&st[4 * (size_t)k], w[k]->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < n; ++k) {
-            SMIP_HIP(hipMemcpyAsync(st.data(), w[k]->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
-            SMIP_HIP(hipStreamSynchronize(s));
-            status[k] = st[0]; // per job: an out-of-range lookup stops that job only
-            calls[k] = st[1];
+            status[k] = st[4 * (size_t)k]; // per job: an out-of-range lookup stops that job only
+            calls[k] = st[4 * (size_t)k + 1];
         }
     } catch (...) {
         dev_free(dtab);
         throw;
     }
+    IdleRelease idle; // s is synchronised
     dev_free(dtab);
 }
 
@@ -1009,27 +1119,32 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
 void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s)
 {
-    const int32_t v[2] = { wx, wy };
-    RecoWork* w = reco_work_create(d, xsize, ysize, v, 1, s);
+    const int npts = xsize * ysize;
+    const size_t lds = sizeof(uint32_t) * ((npts + 31) / 32);
+    if (lds > 96 * 1024)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for calc_phase");
+    (void)reco_cap(d);
+    // one block: the visit, then the status words
+    const int32_t head[4] = { wx, wy, 0, 0 };
+    void* blk = dev_malloc(512);
     try {
-        const int npts = xsize * ysize;
-        hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w->status);
-        const size_t lds = sizeof(uint32_t) * ((npts + 31) / 32);
-        if (lds > 96 * 1024)
-            fail(SMIP_ERR_INVALID_ARGUMENT, "phase map too large for calc_phase");
-        hipLaunchKernelGGL(k_reco_sequential, dim3(1), dim3(kSeqThreads), lds, s, w->args, w->visits, B, is_f32,
-            phases, flags, consistency, w->status, 1);
+        int2* visit = static_cast<int2*>(blk);
+        int32_t* status = reinterpret_cast<int32_t*>(static_cast<char*>(blk) + 256);
+        SMIP_HIP(hipMemcpyAsync(visit, head, sizeof(int2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, status);
+        hipLaunchKernelGGL(k_reco_sequential, dim3(1), dim3(kSeqThreads), lds, s, reco_args(d, xsize, ysize, 1),
+            visit, B, is_f32, phases, flags, consistency, status, 1);
         SMIP_HIP(hipGetLastError());
         int32_t st[4] = { 0, 0, 0, 0 };
-        SMIP_HIP(hipMemcpyAsync(st, w->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+        SMIP_HIP(hipMemcpyAsync(st, status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
         SMIP_HIP(hipStreamSynchronize(s));
         if (st[0])
             fail(st[0], "calc_phase: bispectrum element address out of bounds");
     } catch (...) {
-        reco_work_destroy(w);
+        dev_free(blk);
         throw;
     }
-    reco_work_destroy(w);
+    dev_free(blk);
 }
 
 } // namespace smip_amd

@@ -6,9 +6,11 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "smip_mi355x.h"
 
@@ -38,9 +40,35 @@ void* dev_malloc(size_t bytes);                 // throws SMIP_ERR_NO_MEMORY
 void dev_free(void* p);                         // null allowed
 void* host_malloc(size_t bytes);                // pinned host memory
 void host_free(void* p);
+// While an IdleRelease lives on a thread, that thread's dev_free / host_free keep
+// the block for reuse instead of handing it back to HIP (whose free waits for the
+// whole device).  Only for blocks no queued work can still touch.
+struct IdleRelease {
+    IdleRelease();
+    ~IdleRelease();
+    IdleRelease(const IdleRelease&) = delete;
+    IdleRelease& operator=(const IdleRelease&) = delete;
+};
+void cache_trim(int device);          // return the device's cached blocks to HIP
+size_t cache_device_bytes(int device); // bytes cached for reuse on the device
 hipStream_t make_stream();                      // non-blocking, default priority
 hipError_t mem_get_info(size_t* free_b, size_t* total_b);
 void destroy_stream(hipStream_t s);
+// a stream for phase recovery with a hardware queue of its own (a CU-masked
+// stream from a per-device pool), or `fallback` when none is free
+void reco_streams_release(int device); // destroy the device's idle recovery streams
+class RecoStreamLease {
+  public:
+    RecoStreamLease(int device, hipStream_t fallback);
+    ~RecoStreamLease();
+    RecoStreamLease(const RecoStreamLease&) = delete;
+    RecoStreamLease& operator=(const RecoStreamLease&) = delete;
+    hipStream_t get() const { return use_; }
+
+  private:
+    int dev_;
+    hipStream_t own_ = nullptr, use_;
+};
 // synchronous copy through stream s (which is synchronised)
 void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 
@@ -263,10 +291,18 @@ void launch_abs_max_normalize(double2* data, int64_t n, double* scratch, hipStre
 void launch_power_expand(const double* power_half, int32_t ncols, int32_t nrows, double norm, double* out,
     hipStream_t s);
 
-// phase reconstruction (device); visits = in-range spiral visits (x,y) pairs
+// phase reconstruction (device)
+// in-range spiral visits of reconstruct_phases as (x, y) pairs, host side, cached
+std::shared_ptr<const std::vector<int32_t>> visit_list(double reco_radius, int64_t X, int64_t Y);
+// the visit list on the current device, cached per (device, radius, X, Y); the
+// upload runs on `stream` (synchronously) the first time a geometry is asked for
+struct RecoGeom;
+std::shared_ptr<const RecoGeom> reco_geom(double reco_radius, int32_t xsize, int32_t ysize, hipStream_t stream);
+uint64_t reco_geom_bytes_estimate(double reco_radius, int32_t xsize, int32_t ysize);
+// per-recovery scratch (one device allocation, nothing uploaded)
 struct RecoWork;
-RecoWork* reco_work_create(const Desc& d, int32_t xsize, int32_t ysize, const int32_t* visits_host, int64_t nvisits,
-    hipStream_t stream);
+RecoWork* reco_work_create(const Desc& d, std::shared_ptr<const RecoGeom> geom);
+uint64_t reco_scratch_bytes(const Desc& d, int32_t xsize, int32_t ysize);
 void reco_work_destroy(RecoWork*);
 // n recoveries together (one launch per stage and window index); calls[k] = job k's
 // calc_phase count, status[k] = its status
