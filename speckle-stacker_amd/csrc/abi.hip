@@ -433,7 +433,7 @@ struct smip_job {
     dptr<float2> H32, work32;
     dptr<double2> H64, work64;
     // shift-and-add
-    dptr<double2> ref_spec, sspec, swork;
+    dptr<double2> ref_spec, swork;
     dptr<double> corr, sum;
     dptr<int32_t> shifts;
     uint64_t shifts_cap = 0;
@@ -1352,7 +1352,7 @@ JobSizes job_sizes(const smip_job_params& p)
         b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
     if (p.saa)
         b += hx * Y * sizeof(double2)
-            + (uint64_t)std::min(z.batch, kSaaChunk) * (2 * hx * Y * sizeof(double2) + np * sizeof(double))
+            + (uint64_t)std::min(z.batch, kSaaChunk) * (hx * Y * sizeof(double2) + Y * sizeof(double))
             + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
     b += 2 * (uint64_t)z.batch * np * dtype_size(p.frame_dtype);         // two host staging buffers
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
@@ -1418,9 +1418,8 @@ void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh, hipStre
     for (int64_t o = 0; o < m; o += kSaaChunk) {
         const int64_t c = std::min<int64_t>(kSaaChunk, m - o);
         const uint8_t* f = fm + o * fb;
-        launch_rfft_f64(j->plan, f, p.frame_dtype, c, true, j->sspec.get(), j->swork.get(), s);
         // the correlation buffer (c N^2 doubles) holds the per-workgroup maxima
-        launch_xcorr_shift_f64(j->plan, j->ref_spec.get(), j->sspec.get(), c,
+        launch_shift_f64(j->plan, f, p.frame_dtype, c, true, j->ref_spec.get(),
             reinterpret_cast<RowBest*>(j->corr.get()), j->swork.get(), sh + 2 * o, s);
         launch_saa_add(f, p.frame_dtype, c, X, Y, sh + 2 * o, j->sum.get(), s);
     }
@@ -1690,9 +1689,9 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         if (p->saa) {
             j->ref_spec = dmake<double2>(hx * Y);
             const size_t sc = (size_t)std::min(j->batch, kSaaChunk);
-            j->sspec = dmake<double2>(sc * hx * Y);
             j->swork = dmake<double2>(sc * hx * Y);
-            j->corr = dmake<double>(sc * np);
+            // per-workgroup correlation maxima: at most nrows / 2 RowBest (16 B) per frame
+            j->corr = dmake<double>(sc * Y);
             j->sum = dmake<double>(np);
             SMIP_HIP(hipMemsetAsync(j->sum.get(), 0, sizeof(double) * np, s));
             j->bshift = dmake<int32_t>(2 * (size_t)j->batch);

@@ -414,6 +414,59 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
     }
 }
 
+// The column passes of the cross-correlation in one launch: the forward
+// transform of each column of G (as k_cols_fwd in MODE 1), C = conj(Rref) * R per
+// element (as k_xcorr_cols_inv), and the inverse transform, written back over G.
+// The spectrum stays in LDS: one read and one write of G instead of two each,
+// with the same arithmetic in the same order as the two separate passes.
+__global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restrict__ Rref, int X, int Y, int ly,
+    int hx, int cb, const double2* __restrict__ tw, double2* __restrict__ G)
+{
+    constexpr int kPer = kColLds / (int)sizeof(double2) / kThreads; // elements per thread at cb * Y <= kColLds / 16
+    extern __shared__ char smem[];
+    double2* A = reinterpret_cast<double2*>(smem);
+    const int64_t f = blockIdx.y;
+    const int kx0 = blockIdx.x * cb;
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        const int kx = kx0 + c;
+        A[c * Y + y] = kx < hx ? G[(f * Y + y) * hx + kx] : mk<double2>(0, 0);
+    }
+    __syncthreads();
+    lds_fft_dif<double2>(A, Y, ly, cb, tw, false);
+    // back to natural order through registers, multiplied on the way
+    double2 v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int t = threadIdx.x + q * kThreads;
+        v[q] = mk<double2>(0, 0);
+        if (t < cb * Y) {
+            const int c = t / Y, ky = t - c * Y;
+            const int kx = kx0 + c;
+            if (kx < hx) {
+                const double2 a = Rref[(int64_t)ky * hx + kx];
+                // std::conj(a) * b with the (ac - bd, ad + bc) operand order
+                v[q] = cmul(cconj(a), A[c * Y + dif_pos(ky, ly)]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int t = threadIdx.x + q * kThreads;
+        if (t < cb * Y)
+            A[t] = v[q];
+    }
+    __syncthreads();
+    lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
+    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
+        const int y = t / cb, c = t - y * cb;
+        const int kx = kx0 + c;
+        if (kx < hx)
+            G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
+    }
+}
+
 // Row pass of a c2r transform, two rows at a time: z = IDFT(H0 + i H1) gives
 // row0 = Re z, row1 = Im z.  The imaginary parts of the kx = 0 and kx = X/2 terms
 // are dropped, as a c2r transform of Hermitian data does.
@@ -709,13 +762,14 @@ void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf
     launch_cols_fwd<double2, 1, double2>(p, work, nf, out, hp, s);
 }
 
-void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
-    RowBest* best, double2* work, int32_t* shifts, hipStream_t s)
+void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const double2* ref_spec, RowBest* best, double2* work, int32_t* shifts, hipStream_t s)
 {
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
-    const int cb = cols_per_wg<double2>(Y);
-    hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+    launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
+    const int cb = cols_per_wg<double2>(Y); // cb * Y * 16 <= kColLds, as k_xcorr_cols's registers assume
+    hipLaunchKernelGGL(k_xcorr_cols, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
+        sizeof(double2) * cb * Y, s, ref_spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;

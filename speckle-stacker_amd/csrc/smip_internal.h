@@ -245,10 +245,11 @@ struct RowBest {
     double v;
     int64_t i;
 };
-// The cross-correlation of each spectrum with the reference, reduced straight to
-// its displacement: shifts[2f], shifts[2f+1].  `best` holds nf * nrows / 2 entries.
-void launch_xcorr_shift_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
-    RowBest* best, double2* work, int32_t* shifts, hipStream_t s);
+// The cross-correlation of each of nf frames with the reference spectrum,
+// reduced straight to its displacement: shifts[2f], shifts[2f+1].  work holds
+// nf * nrows * (ncols/2 + 1) complex values; best nf * nrows / 2 entries.
+void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
+    const double2* ref_spec, RowBest* best, double2* work, int32_t* shifts, hipStream_t s);
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
     double* corr, double2* work, hipStream_t s);
 // unnormalised backward complex 2-D FFT, FP64, in place on nrows x ncols.
