@@ -313,6 +313,23 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ fr
     }
 }
 
+// Column blocks of one row segment share 128-byte lines (a block holds 64 bytes
+// of a row at the default tile), and workgroups go to the 8 XCDs, each with its
+// own L2, round-robin in dispatch order.  The dispatch index is remapped so that
+// each XCD takes one contiguous run of (frame, column block) indices: neighbours
+// that share lines then share an L2.  A bijection on the first multiple of 8
+// indices, the identity on the rest.
+__device__ inline void xcd_tile(int& bx, int64_t& f)
+{
+    const uint32_t gx = gridDim.x, n = gx * gridDim.y;
+    uint32_t d = blockIdx.x + blockIdx.y * gx;
+    const uint32_t full = n & ~7u;
+    if (d < full)
+        d = (d & 7u) * (full >> 3) + (d >> 3);
+    bx = (int)(d % gx);
+    f = (int64_t)(d / gx);
+}
+
 // ---------------------------------------------------------------------------
 // Column pass: G[f][y][kx] -> transform along y.
 //   MODE 0: half-plane layout H[f][X/2 - kx][pad + r] = conj(R[(Y/2 - r) mod Y][kx])
@@ -325,8 +342,10 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
 {
     extern __shared__ char smem[];
     V* A = reinterpret_cast<V*>(smem);
-    const int64_t f = blockIdx.y;
-    const int kx0 = blockIdx.x * cb;
+    int bx;
+    int64_t f;
+    xcd_tile(bx, f);
+    const int kx0 = bx * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         const int kx = kx0 + c;
@@ -390,8 +409,10 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
 {
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
-    const int64_t f = blockIdx.y;
-    const int kx0 = blockIdx.x * cb;
+    int bx;
+    int64_t f;
+    xcd_tile(bx, f);
+    const int kx0 = bx * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         const int kx = kx0 + c;
@@ -425,8 +446,10 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
     constexpr int kPer = kColLds / (int)sizeof(double2) / kThreads; // elements per thread at cb * Y <= kColLds / 16
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
-    const int64_t f = blockIdx.y;
-    const int kx0 = blockIdx.x * cb;
+    int bx;
+    int64_t f;
+    xcd_tile(bx, f);
+    const int kx0 = bx * cb;
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t / cb, c = t - y * cb;
         const int kx = kx0 + c;
