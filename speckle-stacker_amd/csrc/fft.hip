@@ -279,6 +279,34 @@ constexpr int kLdsBytes = 65536;
 // at 128x128 13% slower, elsewhere within 2%.
 constexpr int kColLds = 32768;
 constexpr int kThreads = 256;
+// elements per thread of a column tile (cb * Y * sizeof(V) <= kColLds) and of a
+// row tile (ppw * X <= 2048, see rows_per_wg)
+template <typename V>
+constexpr int kColPer = kColLds / (int)sizeof(V) / kThreads;
+constexpr int kRowElems = 2048;
+constexpr int kRowPer = kRowElems / kThreads;
+
+// A workgroup's tile load, issued all at once.  A loop that stores each element
+// to LDS as it arrives waits one full memory round trip per iteration; loading
+// up to KPER elements per thread into registers first keeps them all in flight.
+// load(t) and store(t, v) are called for t = threadIdx.x + q * kThreads < total.
+template <int KPER, typename T, typename Load, typename Store>
+__device__ inline void gather_tile(int total, Load&& load, Store&& store)
+{
+    T r[KPER];
+#pragma unroll
+    for (int q = 0; q < KPER; ++q) {
+        const int t = (int)threadIdx.x + q * kThreads;
+        if (t < total)
+            r[q] = load(t);
+    }
+#pragma unroll
+    for (int q = 0; q < KPER; ++q) {
+        const int t = (int)threadIdx.x + q * kThreads;
+        if (t < total)
+            store(t, r[q]);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Row pass: real rows y0 = 2p, y1 = 2p+1 of frame f -> G[f][y][kx], kx in [0, X/2]
@@ -294,11 +322,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ fr
     const int64_t f = blockIdx.y;
     const int pair0 = blockIdx.x * ppw;
     const In* fr = frames + f * (int64_t)X * Y;
-    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
-        const int p = t >> lx, x = t & (X - 1);
-        const int64_t y0 = 2 * (pair0 + p);
-        A[t] = mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
-    }
+    gather_tile<kRowPer, V>(
+        ppw * X,
+        [&](int t) {
+            const int p = t >> lx, x = t & (X - 1);
+            const int64_t y0 = 2 * (pair0 + p);
+            return mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
+        },
+        [&](int t, const V& v) { A[t] = v; });
     __syncthreads();
     const V* R = lds_fft<V>(A, B, X, lx, ppw, tw, false);
     const S half = (S)0.5;
@@ -346,11 +377,17 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
     int64_t f;
     xcd_tile(bx, f);
     const int kx0 = bx * cb;
-    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
-        const int kx = kx0 + c;
-        A[c * Y + y] = kx < hx ? G[(f * Y + y) * hx + kx] : mk<V>(0, 0);
-    }
+    gather_tile<kColPer<V>, V>(
+        cb * Y,
+        [&](int t) {
+            const int y = t / cb, c = t - y * cb;
+            const int kx = kx0 + c;
+            return kx < hx ? G[(f * Y + y) * hx + kx] : mk<V>(0, 0);
+        },
+        [&](int t, const V& v) {
+            const int y = t / cb, c = t - y * cb;
+            A[c * Y + y] = v;
+        });
     __syncthreads();
     lds_fft_dif<V>(A, Y, ly, cb, tw, false);
     const V* R = A; // digit-reversed: X[k] at dif_pos(k)
@@ -413,18 +450,24 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
     int64_t f;
     xcd_tile(bx, f);
     const int kx0 = bx * cb;
-    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
-        const int kx = kx0 + c;
-        double2 v = mk<double2>(0, 0);
-        if (kx < hx) {
-            const double2 a = Rref[(int64_t)y * hx + kx];
-            const double2 b = R[(f * Y + y) * hx + kx];
-            // std::conj(a) * b with the (ac - bd, ad + bc) operand order
-            v = cmul(cconj(a), b);
-        }
-        A[c * Y + y] = v;
-    }
+    gather_tile<kColPer<double2>, double2>(
+        cb * Y,
+        [&](int t) {
+            const int y = t / cb, c = t - y * cb;
+            const int kx = kx0 + c;
+            double2 v = mk<double2>(0, 0);
+            if (kx < hx) {
+                const double2 a = Rref[(int64_t)y * hx + kx];
+                const double2 b = R[(f * Y + y) * hx + kx];
+                // std::conj(a) * b with the (ac - bd, ad + bc) operand order
+                v = cmul(cconj(a), b);
+            }
+            return v;
+        },
+        [&](int t, const double2& v) {
+            const int y = t / cb, c = t - y * cb;
+            A[c * Y + y] = v;
+        });
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -443,18 +486,24 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
 __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restrict__ Rref, int X, int Y, int ly,
     int hx, int cb, const double2* __restrict__ tw, double2* __restrict__ G)
 {
-    constexpr int kPer = kColLds / (int)sizeof(double2) / kThreads; // elements per thread at cb * Y <= kColLds / 16
+    constexpr int kPer = kColPer<double2>;
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     int bx;
     int64_t f;
     xcd_tile(bx, f);
     const int kx0 = bx * cb;
-    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
-        const int kx = kx0 + c;
-        A[c * Y + y] = kx < hx ? G[(f * Y + y) * hx + kx] : mk<double2>(0, 0);
-    }
+    gather_tile<kPer, double2>(
+        cb * Y,
+        [&](int t) {
+            const int y = t / cb, c = t - y * cb;
+            const int kx = kx0 + c;
+            return kx < hx ? G[(f * Y + y) * hx + kx] : mk<double2>(0, 0);
+        },
+        [&](int t, const double2& v) {
+            const int y = t / cb, c = t - y * cb;
+            A[c * Y + y] = v;
+        });
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, false);
     // back to natural order through registers, multiplied on the way
@@ -501,25 +550,28 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict
     double2* B = A + ppw * X;
     const int64_t f = blockIdx.y;
     const int pair0 = blockIdx.x * ppw;
-    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
-        const int p = t >> lx, k = t & (X - 1);
-        const int64_t y0 = 2 * (pair0 + p);
-        const double2* g0 = G + (f * Y + y0) * hx;
-        const double2* g1 = g0 + hx;
-        double2 h0, h1;
-        if (k <= X / 2) {
-            h0 = g0[k];
-            h1 = g1[k];
-            if (k == 0 || k == X / 2) {
-                h0.y = 0.0;
-                h1.y = 0.0;
+    gather_tile<kRowPer, double2>(
+        ppw * X,
+        [&](int t) {
+            const int p = t >> lx, k = t & (X - 1);
+            const int64_t y0 = 2 * (pair0 + p);
+            const double2* g0 = G + (f * Y + y0) * hx;
+            const double2* g1 = g0 + hx;
+            double2 h0, h1;
+            if (k <= X / 2) {
+                h0 = g0[k];
+                h1 = g1[k];
+                if (k == 0 || k == X / 2) {
+                    h0.y = 0.0;
+                    h1.y = 0.0;
+                }
+            } else {
+                h0 = cconj(g0[X - k]);
+                h1 = cconj(g1[X - k]);
             }
-        } else {
-            h0 = cconj(g0[X - k]);
-            h1 = cconj(g1[X - k]);
-        }
-        A[t] = cadd(h0, cmul_pi(h1));
-    }
+            return cadd(h0, cmul_pi(h1));
+        },
+        [&](int t, const double2& v) { A[t] = v; });
     __syncthreads();
     const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
     for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
@@ -553,25 +605,28 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __r
     double2* B = A + ppw * X;
     const int64_t f = blockIdx.y;
     const int pair0 = blockIdx.x * ppw;
-    for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
-        const int p = t >> lx, k = t & (X - 1);
-        const int64_t y0 = 2 * (pair0 + p);
-        const double2* g0 = G + (f * Y + y0) * hx;
-        const double2* g1 = g0 + hx;
-        double2 h0, h1;
-        if (k <= X / 2) {
-            h0 = g0[k];
-            h1 = g1[k];
-            if (k == 0 || k == X / 2) {
-                h0.y = 0.0;
-                h1.y = 0.0;
+    gather_tile<kRowPer, double2>(
+        ppw * X,
+        [&](int t) {
+            const int p = t >> lx, k = t & (X - 1);
+            const int64_t y0 = 2 * (pair0 + p);
+            const double2* g0 = G + (f * Y + y0) * hx;
+            const double2* g1 = g0 + hx;
+            double2 h0, h1;
+            if (k <= X / 2) {
+                h0 = g0[k];
+                h1 = g1[k];
+                if (k == 0 || k == X / 2) {
+                    h0.y = 0.0;
+                    h1.y = 0.0;
+                }
+            } else {
+                h0 = cconj(g0[X - k]);
+                h1 = cconj(g1[X - k]);
             }
-        } else {
-            h0 = cconj(g0[X - k]);
-            h1 = cconj(g1[X - k]);
-        }
-        A[t] = cadd(h0, cmul_pi(h1));
-    }
+            return cadd(h0, cmul_pi(h1));
+        },
+        [&](int t, const double2& v) { A[t] = v; });
     __syncthreads();
     const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
     // the scan starts from element 0 (max_element's first candidate) in the
@@ -637,8 +692,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2c(double2* __restrict__ dat
     double2* A = reinterpret_cast<double2*>(smem);
     double2* B = A + rpw * X;
     double2* base = data + (int64_t)blockIdx.x * rpw * X;
-    for (int t = threadIdx.x; t < rpw * X; t += blockDim.x)
-        A[t] = base[t];
+    gather_tile<kRowPer, double2>(
+        rpw * X, [&](int t) { return base[t]; }, [&](int t, const double2& v) { A[t] = v; });
     __syncthreads();
     const double2* Z = lds_fft<double2>(A, B, X, lx, rpw, tw, inv != 0);
     for (int t = threadIdx.x; t < rpw * X; t += blockDim.x)
@@ -651,10 +706,16 @@ __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ dat
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     const int x0 = blockIdx.x * cb;
-    for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
-        A[c * Y + y] = (x0 + c < X) ? data[(int64_t)y * X + x0 + c] : mk<double2>(0, 0);
-    }
+    gather_tile<kColPer<double2>, double2>(
+        cb * Y,
+        [&](int t) {
+            const int y = t / cb, c = t - y * cb;
+            return (x0 + c < X) ? data[(int64_t)y * X + x0 + c] : mk<double2>(0, 0);
+        },
+        [&](int t, const double2& v) {
+            const int y = t / cb, c = t - y * cb;
+            A[c * Y + y] = v;
+        });
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, inv != 0);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -705,7 +766,10 @@ template <typename V>
 static int rows_per_wg(int n, int cap)
 {
     const int fill = std::max(1, (4 * kThreads) / n);
-    return std::min(per_wg<V>(n, cap), fill);
+    const int r = std::min(per_wg<V>(n, cap), fill);
+    if (r * n > kRowElems) // the row kernels' tile loads hold kRowPer elements per thread
+        fail(SMIP_ERR_INVALID_ARGUMENT, "row tile exceeds the per-thread load budget");
+    return r;
 }
 
 template <typename V>
