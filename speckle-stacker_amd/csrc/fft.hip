@@ -599,7 +599,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict
 // k_rows_c2r with calculate_displacement's search (include/crosscorrel.h:166-179)
 // fused in: instead of writing its correlation rows, a workgroup writes the
 // first maximum of its rows in row-major order to best[f][blockIdx.x], and
-// k_best_to_shift merges those in order.  The correlation never reaches memory.
+// k_best_to_shift merges those into the frame's first maximum.  The correlation
+// never reaches memory.
 __device__ inline void better_of(double& bv, int64_t& bi, double v, int64_t i)
 {
     // (v, i) replaces (bv, bi) when it is larger, or equal with a lower index
@@ -678,17 +679,29 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __r
 
 // merges the workgroups' maxima of each frame in index order and applies the
 // signed wrap x >= (n+1)/2 -> x - n
-__global__ void k_best_to_shift(const RowBest* __restrict__ best, int nblk, int64_t nf, int X, int Y,
-    int32_t* __restrict__ shifts)
+__global__ __launch_bounds__(64) void k_best_to_shift(const RowBest* __restrict__ best, int nblk, int64_t nf, int X,
+    int Y, int32_t* __restrict__ shifts)
 {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per frame: each lane scans every 64th entry, then the lanes'
+    // results merge; better_of keeps the larger value and, on ties, the lower
+    // index, so the order of the merge does not change the result
+    const int64_t f = blockIdx.x;
+    const int lane = threadIdx.x;
     if (f >= nf)
         return;
     const RowBest* b = best + f * nblk;
-    double bv = b[0].v;
-    int64_t bi = b[0].i;
-    for (int k = 1; k < nblk; ++k)
+    double bv = -INFINITY;
+    int64_t bi = INT64_MAX;
+    for (int k = lane; k < nblk; k += 64)
         better_of(bv, bi, b[k].v, b[k].i);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o);
+        const int64_t oi = __shfl_xor(bi, o);
+        better_of(bv, bi, ov, oi);
+    }
+    if (lane != 0)
+        return;
     int32_t sx = (int32_t)(bi % X), sy = (int32_t)(bi / X);
     if (!(sx < (X + 1) / 2))
         sx -= X;
@@ -877,7 +890,7 @@ void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t n
     hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
         s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
     SMIP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_best_to_shift, dim3((unsigned)((nf + 63) / 64)), dim3(64), 0, s, best, nblk, nf, X, Y, shifts);
+    hipLaunchKernelGGL(k_best_to_shift, dim3((unsigned)nf), dim3(64), 0, s, best, nblk, nf, X, Y, shifts);
     SMIP_HIP(hipGetLastError());
 }
 

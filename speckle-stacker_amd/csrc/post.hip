@@ -107,14 +107,14 @@ __global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, 
     const int32_t col = (int32_t)(p % X), row = (int32_t)(p / X);
     double acc = sum[p];
     int64_t f = 0;
-    // eight frames' loads in flight at once, added in frame order
-    for (; f + 8 <= nf; f += 8) {
-        double v[8];
+    // sixteen frames' loads in flight at once, added in frame order
+    for (; f + 16 <= nf; f += 16) {
+        double v[16];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < 16; ++k)
             v[k] = shifted_abs(frames, f + k, X, Y, col, row, shifts);
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < 16; ++k)
             acc += v[k];
     }
     for (; f < nf; ++f)
