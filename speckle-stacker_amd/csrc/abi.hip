@@ -251,77 +251,6 @@ hipStream_t make_stream()
     return s;
 }
 
-// Streams for phase recovery.  A recovery is a chain of hundreds of short
-// dependent launches.  HIP maps a process's streams onto a few hardware queues,
-// and on a queue shared with other jobs' streams their batches wait behind the
-// whole chain.  A stream created with a CU mask gets a hardware queue of its own;
-// the mask here names every CU.  Kept per device for reuse, at most 8 per device;
-// beyond that (or where such a stream cannot be made) the caller's stream is used.
-namespace {
-struct RecoStreamPool {
-    std::mutex mu;
-    std::map<int, std::vector<hipStream_t>> idle;
-    std::map<int, int> made;
-    static RecoStreamPool& get()
-    {
-        static auto* p = new RecoStreamPool; // never destroyed: streams live to the end of the process
-        return *p;
-    }
-};
-} // namespace
-
-RecoStreamLease::RecoStreamLease(int dev, hipStream_t fallback)
-    : dev_(dev)
-    , use_(fallback)
-{
-    RecoStreamPool& P = RecoStreamPool::get();
-    std::lock_guard<std::mutex> lk(P.mu);
-    auto& v = P.idle[dev];
-    if (!v.empty()) {
-        own_ = v.back();
-        v.pop_back();
-    } else if (P.made[dev] < 8) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) {
-            const int ncu = prop.multiProcessorCount;
-            std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
-            if (ncu % 32)
-                mask.back() = (1u << (ncu % 32)) - 1u;
-            std::lock_guard<std::recursive_mutex> lk2(capture_guard());
-            hipStream_t st = nullptr;
-            if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess) {
-                own_ = st;
-                ++P.made[dev];
-            }
-        }
-        (void)hipGetLastError();
-    }
-    if (own_)
-        use_ = own_;
-}
-
-void reco_streams_release(int dev)
-{
-    RecoStreamPool& P = RecoStreamPool::get();
-    std::vector<hipStream_t> v;
-    {
-        std::lock_guard<std::mutex> lk(P.mu);
-        v.swap(P.idle[dev]);
-        P.made[dev] -= (int)v.size();
-    }
-    for (hipStream_t st : v)
-        destroy_stream(st);
-}
-
-RecoStreamLease::~RecoStreamLease()
-{
-    if (!own_)
-        return;
-    RecoStreamPool& P = RecoStreamPool::get();
-    std::lock_guard<std::mutex> lk(P.mu);
-    P.idle[dev_].push_back(own_);
-}
-
 void destroy_stream(hipStream_t s)
 {
     if (!s)
@@ -719,7 +648,6 @@ extern "C" SMIP_API int smip_ctx_destroy(smip_ctx* ctx)
         if (!ctx)
             return;
         destroy_stream(ctx->stream);
-        reco_streams_release(ctx->device);
         cache_trim(ctx->device);
         delete ctx;
     });
@@ -1945,15 +1873,9 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
         launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, s);
         hipEvent_t e0 = mark(j);
         check_reco_geometry(j->d, p.ncols, p.nrows);
-        RecoStreamLease lease(j->ctx->device, s);
-        const hipStream_t rs = lease.get();
-        if (rs != s)
-            SMIP_HIP(hipStreamWaitEvent(rs, e0, 0));
         j->reco_calls = (uint64_t)run_reconstruct(j->reco, j->B64.get(), 0, j->phases.get(), j->flags.get(),
-            j->cons.get(), rs);
-        hipEvent_t e1 = mark_on(j, rs);
-        if (rs != s)
-            SMIP_HIP(hipStreamWaitEvent(s, e1, 0));
+            j->cons.get(), s);
+        hipEvent_t e1 = mark(j);
         record(j, 2, e0, e1);
         hipEvent_t e2 = mark(j);
         const double norm = (double)(j->nframes * p.ncols * p.nrows);
@@ -2007,8 +1929,7 @@ extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n
             return;
         }
         ensure_device(jobs[0]->ctx);
-        RecoStreamLease lease(jobs[0]->ctx->device, jobs[0]->stream);
-        hipStream_t cs = lease.get(); // the group's stream
+        hipStream_t cs = jobs[0]->stream; // the group's stream
         std::vector<hipEvent_t> ev;
         std::vector<RecoWork*> work;
         auto cleanup = [&] {

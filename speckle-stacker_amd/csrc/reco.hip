@@ -41,7 +41,7 @@ namespace smip_amd {
 
 constexpr double kEps = 2.220446049250313e-16; // constants::c_epsilon<double>
 constexpr int kWindow = 256;                   // visits per window
-constexpr int kBulkThreads = 256;
+constexpr int kBulkThreads = 1024;            // lanes per visit in the bulk pass (latency-bound loads)
 constexpr int kFixBuf = 512; // pending records staged in LDS per chunk
 
 struct RecoArgs {
@@ -841,7 +841,7 @@ static int reco_cap(const Desc& d)
 {
     const int ncand = (d.max_idx[2] - d.min_idx[2] + 1) * (d.max_idx[3] - d.min_idx[3] + 1);
     if (ncand > 64 * kBulkThreads)
-        fail(SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth too large for phase recovery (b/2 <= 63)");
+        fail(SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth too large for phase recovery ((b+1)^2 <= 65536)");
     return std::min(ncand, 2 * kWindow);
 }
 

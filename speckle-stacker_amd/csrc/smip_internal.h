@@ -54,21 +54,6 @@ size_t cache_device_bytes(int device); // bytes cached for reuse on the device
 hipStream_t make_stream();                      // non-blocking, default priority
 hipError_t mem_get_info(size_t* free_b, size_t* total_b);
 void destroy_stream(hipStream_t s);
-// a stream for phase recovery with a hardware queue of its own (a CU-masked
-// stream from a per-device pool), or `fallback` when none is free
-void reco_streams_release(int device); // destroy the device's idle recovery streams
-class RecoStreamLease {
-  public:
-    RecoStreamLease(int device, hipStream_t fallback);
-    ~RecoStreamLease();
-    RecoStreamLease(const RecoStreamLease&) = delete;
-    RecoStreamLease& operator=(const RecoStreamLease&) = delete;
-    hipStream_t get() const { return use_; }
-
-  private:
-    int dev_;
-    hipStream_t own_ = nullptr, use_;
-};
 // synchronous copy through stream s (which is synchronised)
 void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 
