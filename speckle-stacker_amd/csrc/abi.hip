@@ -251,6 +251,30 @@ hipStream_t make_stream()
     return s;
 }
 
+// A stream with a hardware queue of its own: HIP maps a process's ordinary streams
+// onto a few shared queues (four by default), where a phase recovery's long chain
+// of short launches holds up every other stream on its queue.  A stream created
+// with a CU mask gets a queue of its own; the mask here names every CU.  Falls
+// back to an ordinary stream where such a stream cannot be made.
+hipStream_t make_queue_stream()
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess
+        && prop.multiProcessorCount > 0) {
+        const int ncu = prop.multiProcessorCount;
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
+        if (ncu % 32)
+            mask.back() = (1u << (ncu % 32)) - 1u;
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        hipStream_t s = nullptr;
+        if (hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()) == hipSuccess)
+            return s;
+    }
+    (void)hipGetLastError();
+    return make_stream();
+}
+
 void destroy_stream(hipStream_t s)
 {
     if (!s)
@@ -1892,6 +1916,13 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
 
 extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n, int32_t* status)
 {
+    return smip_amd::job_finalize_many_on(jobs, n, status, nullptr);
+}
+
+// smip_job_finalize_many with the group's phase recovery on `group_stream`
+// (nullptr: the first job's stream)
+int smip_amd::job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* status, hipStream_t group_stream)
+{
     int first_err = SMIP_OK;
     std::string first_msg;
     const int rc = guarded([&] {
@@ -1929,7 +1960,7 @@ extern "C" SMIP_API int smip_job_finalize_many(smip_job* const* jobs, uint64_t n
             return;
         }
         ensure_device(jobs[0]->ctx);
-        hipStream_t cs = jobs[0]->stream; // the group's stream
+        hipStream_t cs = group_stream ? group_stream : jobs[0]->stream; // the group's stream
         std::vector<hipEvent_t> ev;
         std::vector<RecoWork*> work;
         auto cleanup = [&] {

@@ -69,6 +69,10 @@ struct smip_server {
     uint64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_failed = 0, frames = 0;
     uint64_t n_groups = 0, n_grouped = 0; // finalisation groups and the jobs they held
     std::vector<std::thread> workers;
+    // one stream per worker with a hardware queue of its own, for the phase
+    // recoveries of the groups it finalises: on a queue shared with the jobs'
+    // streams, a recovery's chain of launches would hold up their batches
+    std::vector<hipStream_t> qstreams;
 
     // first-fit admission in arrival order; caller holds m
     void admit_locked()
@@ -137,7 +141,7 @@ struct smip_server {
         cv_work.notify_all();
     }
 
-    void worker_loop()
+    void worker_loop(int w)
     {
         std::unique_lock<std::mutex> lk(m);
         for (;;) {
@@ -180,7 +184,7 @@ struct smip_server {
                 std::vector<smip_job*> handles;
                 for (ServedJob* g : group)
                     handles.push_back(g->job);
-                rc = smip_job_finalize_many(handles.data(), handles.size(), gst.data());
+                rc = smip_amd::job_finalize_many_on(handles.data(), handles.size(), gst.data(), qstreams[w]);
                 if (rc != SMIP_OK && std::all_of(gst.begin(), gst.end(), [](int32_t c) { return c == SMIP_OK; }))
                     std::fill(gst.begin(), gst.end(), rc); // the group as a whole failed
             } else {
@@ -250,9 +254,12 @@ extern "C" SMIP_API int smip_server_create(smip_ctx* ctx, const smip_server_para
         const int nw = q.workers > 0 ? q.workers : 8;
         if (nw > 64 || s->max_active > 4096)
             fail(SMIP_ERR_INVALID_ARGUMENT, "server parameters out of range");
+        smip_amd::ensure_device(ctx);
+        for (int w = 0; w < nw; ++w)
+            s->qstreams.push_back(smip_amd::make_queue_stream());
         smip_server* raw = s.get();
         for (int w = 0; w < nw; ++w)
-            s->workers.emplace_back([raw] { raw->worker_loop(); });
+            s->workers.emplace_back([raw, w] { raw->worker_loop(w); });
         *out = s.release();
     });
 }
@@ -272,6 +279,9 @@ extern "C" SMIP_API int smip_server_destroy(smip_server* s)
         for (auto& kv : s->jobs)
             if (kv.second->job)
                 smip_job_destroy(kv.second->job);
+        smip_amd::ensure_device(s->ctx);
+        for (hipStream_t q : s->qstreams)
+            smip_amd::destroy_stream(q);
         delete s;
     });
 }

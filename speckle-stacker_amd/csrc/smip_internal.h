@@ -52,6 +52,7 @@ struct IdleRelease {
 void cache_trim(int device);          // return the device's cached blocks to HIP
 size_t cache_device_bytes(int device); // bytes cached for reuse on the device
 hipStream_t make_stream();                      // non-blocking, default priority
+hipStream_t make_queue_stream();                // the same, with a hardware queue of its own
 hipError_t mem_get_info(size_t* free_b, size_t* total_b);
 void destroy_stream(hipStream_t s);
 // synchronous copy through stream s (which is synchronised)
@@ -315,4 +316,7 @@ struct smip_ctx {
 
 namespace smip_amd {
 inline void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }
+// smip_job_finalize_many with the group's phase recovery on group_stream (nullptr:
+// the first job's stream); the server passes each worker's own-queue stream
+int job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* status, hipStream_t group_stream);
 } // namespace smip_amd
