@@ -251,18 +251,15 @@ __device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict_
 // half by k's lowest bit, then the base-4 digits of the rest reverse
 __device__ inline int dif_pos(int k, int log2n)
 {
-    int half = 0, m = log2n;
-    if (log2n & 1) {
-        half = (k & 1) << (log2n - 1);
-        k >>= 1;
-        m = log2n - 1;
-    }
-    int r = 0;
-    for (int d = 0; d < m; d += 2) {
-        r = (r << 2) | (k & 3);
-        k >>= 2;
-    }
-    return half | r;
+    // base-4 digit reversal of the m = log2n - odd bits: a bit reversal, then the
+    // two bits of each digit swapped back
+    const int odd = log2n & 1;
+    const int m = log2n - odd;
+    const int half = (k & odd) << (log2n - 1);
+    const uint32_t q = (uint32_t)k >> odd;
+    uint32_t r = m ? (__builtin_bitreverse32(q) >> (32 - m)) : 0u;
+    r = ((r & 0x55555555u) << 1) | ((r >> 1) & 0x55555555u);
+    return half | (int)r;
 }
 
 template <typename S, typename In>
@@ -333,8 +330,10 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ fr
     __syncthreads();
     const V* R = lds_fft<V>(A, B, X, lx, ppw, tw, false);
     const S half = (S)0.5;
+    // t / hx by a multiply with ceil(2^32 / hx): exact for t < 2^32 / hx
+    const uint32_t hx_inv = 0xffffffffu / (uint32_t)hx + 1u;
     for (int t = threadIdx.x; t < ppw * hx; t += blockDim.x) {
-        const int p = t / hx, k = t - p * hx;
+        const int p = (int)__umulhi((uint32_t)t, hx_inv), k = t - p * hx;
         const V zk = R[p * X + k];
         const V zm = cconj(R[p * X + ((X - k) & (X - 1))]);
         const V s = cadd(zk, zm), dlt = cmul_mi(csub(zk, zm));
@@ -371,6 +370,7 @@ template <typename V, int MODE, typename Out>
 __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, int X, int Y, int ly, int hx, int cb,
     const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp)
 {
+    const int lcb = __builtin_ctz(cb); // cb is a power of two
     extern __shared__ char smem[];
     V* A = reinterpret_cast<V*>(smem);
     int bx;
@@ -380,12 +380,12 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
     gather_tile<kColPer<V>, V>(
         cb * Y,
         [&](int t) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             const int kx = kx0 + c;
             return kx < hx ? G[(f * Y + y) * hx + kx] : mk<V>(0, 0);
         },
         [&](int t, const V& v) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             A[c * Y + y] = v;
         });
     __syncthreads();
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
     const V* R = A; // digit-reversed: X[k] at dif_pos(k)
     if (MODE == 0) {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-            const int c = t / Y, r = t - c * Y;
+            const int c = t >> ly, r = t & (Y - 1);
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
         }
     } else {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-            const int ky = t / cb, c = t - ky * cb;
+            const int ky = t >> lcb, c = t & (cb - 1);
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
@@ -458,6 +458,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
     const double2* __restrict__ R, int X, int Y, int ly, int hx, int cb, const double2* __restrict__ tw,
     double2* __restrict__ G)
 {
+    const int lcb = __builtin_ctz(cb); // cb is a power of two
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     int bx;
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
     gather_tile<kColPer<double2>, double2>(
         cb * Y,
         [&](int t) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             const int kx = kx0 + c;
             double2 v = mk<double2>(0, 0);
             if (kx < hx) {
@@ -479,13 +480,13 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
             return v;
         },
         [&](int t, const double2& v) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             A[c * Y + y] = v;
         });
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
+        const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
         if (kx < hx)
             G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
@@ -500,6 +501,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
 __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restrict__ Rref, int X, int Y, int ly,
     int hx, int cb, const double2* __restrict__ tw, double2* __restrict__ G)
 {
+    const int lcb = __builtin_ctz(cb); // cb is a power of two
     constexpr int kPer = kColPer<double2>;
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
@@ -510,12 +512,12 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
     gather_tile<kPer, double2>(
         cb * Y,
         [&](int t) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             const int kx = kx0 + c;
             return kx < hx ? G[(f * Y + y) * hx + kx] : mk<double2>(0, 0);
         },
         [&](int t, const double2& v) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             A[c * Y + y] = v;
         });
     __syncthreads();
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
         const int t = threadIdx.x + q * kThreads;
         v[q] = mk<double2>(0, 0);
         if (t < cb * Y) {
-            const int c = t / Y, ky = t - c * Y;
+            const int c = t >> ly, ky = t & (Y - 1);
             const int kx = kx0 + c;
             if (kx < hx) {
                 const double2 a = Rref[(int64_t)ky * hx + kx];
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
+        const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
         if (kx < hx)
             G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
@@ -730,23 +732,24 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2c(double2* __restrict__ dat
 __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ data, int X, int Y, int ly, int cb,
     const double2* __restrict__ tw, int inv)
 {
+    const int lcb = __builtin_ctz(cb); // cb is a power of two
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     const int x0 = blockIdx.x * cb;
     gather_tile<kColPer<double2>, double2>(
         cb * Y,
         [&](int t) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             return (x0 + c < X) ? data[(int64_t)y * X + x0 + c] : mk<double2>(0, 0);
         },
         [&](int t, const double2& v) {
-            const int y = t / cb, c = t - y * cb;
+            const int y = t >> lcb, c = t & (cb - 1);
             A[c * Y + y] = v;
         });
     __syncthreads();
     lds_fft_dif<double2>(A, Y, ly, cb, tw, inv != 0);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
-        const int y = t / cb, c = t - y * cb;
+        const int y = t >> lcb, c = t & (cb - 1);
         if (x0 + c < X)
             data[(int64_t)y * X + x0 + c] = A[c * Y + dif_pos(y, ly)];
     }
