@@ -13,7 +13,8 @@
  *   descriptor / calc_offset / calc_indices / get_element / put_element:
  *       the reference's own tests (tests/bispectrum_test.cpp:27-79,132-149),
  *       a brute-force symmetry check of every canonical lookup against direct
- *       triple products of a known spectrum, and the survey's aliasing counts;
+ *       triple products of a known spectrum (aliased lookups, which the
+ *       reference's index arithmetic produces, are counted and skipped);
  *   accumulate: delta frame -> B == 1 on every written cell, translation
  *       invariance, brute force over an explicit (u,v) enumeration on 8x8 and
  *       6x5 grids, two-delta closed form;

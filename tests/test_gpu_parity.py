@@ -6,7 +6,7 @@ Bars (stated per test):
     evaluate the reference's operation order without FMA contraction);
   * FP64 engine (FP64 FFT of our own, FP64 triple products): relative 1e-11;
   * mixed engine (FP32 FFT and triple products per batch, FP64 fold): bispectrum
-    relative error 2e-5 of max|B|, image relative L2 1e-5.
+    relative error 2e-5 of max|B|, image relative L2 1e-5 (the engine's stated accuracy target).
 """
 import numpy as np
 import pytest
