@@ -7,6 +7,10 @@ Arrays follow the oracle's conventions: complex data is ``complex128`` (or
 ``complex64`` for the CLI's ``Bispectrum<complex<float>>``), 2-D arrays are
 ``(nrows, ncols)`` row-major with x fastest, which is the layout of
 ``smip::Array2`` (include/array2.h:290-308).
+
+Parity unpinned (stated in smip_oracle.c's header as well): get_element on the
+aliased T3/T9 lookups, the NextRecoIndex visit order beyond its invariants, and
+displacement maxima that tie to the last bit of the FFT rounding.
 """
 from __future__ import annotations
 

@@ -9,7 +9,7 @@
  * result depends on it (complex products, the calc_phase mean, the spiral) and is
  * compiled with -ffp-contract=off so no FMA contraction changes a rounding.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
+ * Pinning (what checks each function; see tests/test_oracle.py):
  *   descriptor / calc_offset / calc_indices / get_element / put_element:
  *       the reference's own tests (tests/bispectrum_test.cpp:27-79,132-149),
  *       a brute-force symmetry check of every canonical lookup against direct
@@ -26,6 +26,15 @@
  *   xcorr / displacement / shifted: brute-force circular correlation, exact
  *       recovery of integer shifts, tests/array2_test.cpp:329-345;
  *   window: closed form; pipeline: composition of the above.
+ *
+ * Parity unpinned (no reference output can be produced here; each is fixed
+ * only by restating the reference's arithmetic):
+ *   get_element on aliased (T3/T9) lookups: the values follow calc_offset's
+ *       verbatim restatement, with no independent check;
+ *   NextRecoIndex: the visit order beyond its first visits and per-radius
+ *       counts depends on libm cos/sin;
+ *   displacement: maxima that tie to the last bit of the FFT rounding (FFTW is
+ *       absent); the synthetic frames avoid near-ties.
  */
 #include "smip_oracle.h"
 
