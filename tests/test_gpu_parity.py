@@ -103,6 +103,26 @@ def test_reconstruct_phases_dropin(ctx, N, b, p):
     assert np.all(ph[~m] == 0)
 
 
+def test_reconstruct_phases_radius_beyond_the_map(ctx):
+    """Past the map's half-diagonal the spiral (src/phasereco.cpp:17-36) only visits
+    points outside the PhaseMap, which reconstruct_phases skips; the visit list is
+    cut there, so a radius of 1e6 finishes at once and gives the oracle's result at
+    a radius that already covers the map."""
+    N, b = 16, 8
+    frames = _frames(4, N).astype(np.float64)
+    B, _ = O.power_and_bispectrum(frames, b)
+    d = O.desc(N, N, b)
+    ref = O.reconstruct_phases(d, B, N, N, 2.0 * N)
+    gpu = S.Bispectrum(ctx, (N, N, b, b))
+    gpu.upload(B)
+    ph, fl, co = gpu.reconstruct_phases(N, N, 1e6)
+    assert np.array_equal(fl, ref.flags)
+    m = ref.flags.astype(bool)
+    assert m.sum() > N * N // 2
+    assert np.max(np.abs(ph[m] - ref.phases[m])) < 1e-9
+    assert np.max(np.abs(co - ref.consistency)) < 1e-9
+
+
 def test_reconstruct_phases_c64_bispectrum(ctx):
     """The CLI's Bispectrum<complex<float>> input (smip-cli/main.cpp:319)."""
     N, b, p = 32, 8, 16
