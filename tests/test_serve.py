@@ -136,6 +136,26 @@ def test_released_memory_is_reused_without_changing_results(ctx):
     assert free1 >= free0 - (64 << 20)
 
 
+def test_servers_made_and_destroyed_in_turn(ctx):
+    """Each server makes one own-queue recovery stream per worker and destroys them
+    with itself: servers made and destroyed in turn, each finalising a job, keep
+    working and give the same result."""
+    fr = _frames("binary128", 64, 12, seed_offset=9)
+    prm = S.job_params(64, 64, 16, 24, fr.dtype, frames_per_batch=8)
+    outs = []
+    for i in range(6):
+        srv = S.Server(ctx, max_active=2, workers=1 + i % 3)
+        jid = srv.submit(prm)
+        srv.push(jid, fr)
+        srv.close_job(jid)
+        assert srv.wait(jid, 120.0) == S.JOB_DONE
+        outs.append(_outputs(srv.job(jid)))
+        srv.release(jid)
+        srv.close()
+    for o in outs[1:]:
+        _assert_identical(o, outs[0])
+
+
 def test_push_mixing_rules(ctx):
     frames = _frames("binary128", 64, 8)
     job = S.Job(ctx, S.job_params(64, 64, 16, 24, frames.dtype, frames_per_batch=32))
