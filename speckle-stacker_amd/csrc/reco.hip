@@ -469,7 +469,7 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
-    double* __restrict__ cons, int32_t* __restrict__ status)
+    double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
     __shared__ PendingRec buf[2][kChunkRecs]; // double-buffered record staging
     __shared__ StepInfo info[kWindow];
@@ -633,6 +633,16 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
             phases[pw] = ph;
         }
     }
+    // the next window's point map, in place of a launch of its own before the next
+    // bulk pass: nothing in this window reads the map any more
+    if (!aborted) {
+        const int64_t s1 = e, e1 = min(e + (int64_t)kWindow, a.nvisits);
+        for (int64_t k = s1 + lane; k < e1; k += 64)
+            if (first_pos[k] == (int32_t)(k - s1)) {
+                const int2 w = visits[k];
+                map[pm_at(a, w.x, w.y)] = ((win + 1) << 32) | (k - s1);
+            }
+    }
     if (lane == 0) {
         atomicAdd(&status[1], calls);
         atomicAdd(&status[2], flagged);
@@ -658,9 +668,9 @@ __global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __res
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
-    double* __restrict__ cons, int32_t* __restrict__ status)
+    double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
-    reco_fixup_body(a, visits, first_pos, s, e, S, N, E, cnt, rec, cap, flags, phases, cons, status);
+    reco_fixup_body(a, visits, first_pos, s, e, S, N, E, cnt, rec, cap, flags, phases, cons, status, win, map);
 }
 
 // ---------------------------------------------------------------------------
@@ -712,7 +722,7 @@ __global__ __launch_bounds__(64) void k_reco_fixup_many(const RecoJob* __restric
     if (s0 >= a.nvisits)
         return;
     reco_fixup_body(a, J.visits, J.first_pos, s0, min(s0 + kWindow, a.nvisits), J.S, J.N, J.E, J.cnt, J.rec, J.cap,
-        J.flags, J.phases, J.cons, J.status);
+        J.flags, J.phases, J.cons, J.status, win, J.map);
 }
 
 // ---------------------------------------------------------------------------
@@ -1033,12 +1043,13 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
         for (int64_t win = 0; win < w->nwin; ++win) {
             const int64_t s0 = win * kWindow, e0 = std::min<int64_t>(s0 + kWindow, a.nvisits);
             const int nk = (int)(e0 - s0);
-            hipLaunchKernelGGL(k_reco_map, dim3((nk + 255) / 256), dim3(256), 0, s, w->visits, w->first_pos, a, s0,
-                e0, win, w->map);
+            if (win == 0) // later windows' maps are written by the fixup before them
+                hipLaunchKernelGGL(k_reco_map, dim3((nk + 255) / 256), dim3(256), 0, s, w->visits, w->first_pos, a,
+                    s0, e0, win, w->map);
             hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, s0, win, w->map, flags,
                 phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
             hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(64), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
-                w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status);
+                w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status, win, w->map);
         }
         SMIP_HIP(hipGetLastError());
     }
@@ -1093,7 +1104,8 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
     try {
         copy_sync(dtab, tab.data(), sizeof(RecoJob) * n, hipMemcpyHostToDevice, s);
         for (int64_t win = 0; win < maxwin; ++win) {
-            hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
+            if (win == 0) // later windows' maps are written by the fixups before them
+                hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
             hipLaunchKernelGGL(k_reco_bulk_many, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
             hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(64), 0, s, dtab, win);
         }
