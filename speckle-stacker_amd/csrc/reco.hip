@@ -485,20 +485,28 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
         return;
     const int nsteps = (int)(e - s);
     const int nchunks = (nsteps + kChunk - 1) / kChunk;
-    // per-visit scalars of the whole window (independent loads)
-    for (int t = lane; t < nsteps; t += 64) {
-        StepInfo si;
-        si.n0 = N[t];
-        si.fp = first_pos[s + t];
-        si.cnt = cnt[t];
-        si.err = E[t];
-        si.rev = si.fp != t;
-        const int2 w = visits[s + t];
-        si.pw = pm_at(a, w.x, w.y);
-        si.S = S[t];
-        si.off = 0;
-        si.pad = 0;
-        info[t] = si;
+    // per-visit scalars of the whole window: independent loads, all in flight at once
+    {
+        StepInfo sv[kWindow / 64];
+#pragma unroll
+        for (int q = 0; q < kWindow / 64; ++q) {
+            const int t = min(lane + 64 * q, nsteps - 1); // nsteps >= 1; a clamped lane stores nothing
+            StepInfo& si = sv[q];
+            si.n0 = N[t];
+            si.fp = first_pos[s + t];
+            si.cnt = cnt[t];
+            si.err = E[t];
+            si.rev = si.fp != t;
+            const int2 w = visits[s + t];
+            si.pw = pm_at(a, w.x, w.y);
+            si.S = S[t];
+            si.off = 0;
+            si.pad = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kWindow / 64; ++q)
+            if (lane + 64 * q < nsteps)
+                info[lane + 64 * q] = sv[q];
     }
     for (int t = lane; t < kWindow; t += 64)
         ppos[t] = mk<double2>(kNotYet, 0.0);
