@@ -74,6 +74,9 @@ SMIP_API int smip_version(int32_t* major, int32_t* minor, int32_t* patch);
 /* ------------------------------------------------------------------------ */
 typedef struct smip_ctx smip_ctx;
 SMIP_API int smip_ctx_create(int32_t device, smip_ctx** out);
+/* Fails with SMIP_ERR_LOGIC (and destroys nothing) while objects made on the
+ * context -- bispectra, frame stacks, cross-correlations, jobs, servers -- are
+ * alive: each of them uses the context until it is destroyed. */
 SMIP_API int smip_ctx_destroy(smip_ctx* ctx);
 SMIP_API int smip_ctx_synchronize(smip_ctx* ctx);
 SMIP_API int smip_ctx_mem_info(smip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
@@ -227,7 +230,10 @@ SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frames, uint64_t
  * replaced by the autotuner's cached choice for (ncols, nrows, bdepth) when
  * there is one, else by the built-in defaults. */
 SMIP_API int smip_job_resolve_params(const smip_job_params* p, smip_job_params* out);
-/* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376). */
+/* Normalise by n (main.cpp:307-312), recover phases (:319), synthesise (:328-376).
+ * The division by n happens once per job: if phase recovery fails, the job stays
+ * unfinalized with its bispectrum normalised, a later call retries the recovery
+ * on that same bispectrum, and no more frames can be added. */
 SMIP_API int smip_job_finalize(smip_job* j);
 /* smip_job_finalize of n jobs at once (no counterpart in the reference): their
  * phase recoveries run as one group, one launch per stage and window index

@@ -329,6 +329,7 @@ constexpr double k2pi = 3.14159265358979323846 * 2.0; // constants::c_2pi<double
 // objects
 // ---------------------------------------------------------------------------
 struct smip_bispec {
+    CtxRef ctx_ref;
     smip_ctx* ctx = nullptr;
     Desc d {};
     int dtype = SMIP_DTYPE_C128;
@@ -343,6 +344,7 @@ struct smip_bispec {
 };
 
 struct smip_xcorr {
+    CtxRef ctx_ref;
     smip_ctx* ctx = nullptr;
     int X = 0, Y = 0;
     FftPlan* plan = nullptr;
@@ -355,6 +357,7 @@ struct smip_xcorr {
 };
 
 struct smip_frames {
+    CtxRef ctx_ref;
     smip_ctx* ctx = nullptr;
     int dtype = 0;
     uint64_t X = 0, Y = 0, count = 0;
@@ -372,6 +375,7 @@ struct EventPair {
 };
 
 struct smip_job {
+    CtxRef ctx_ref;
     smip_ctx* ctx = nullptr;
     smip_job_params p {};
     hipStream_t stream = nullptr;
@@ -432,6 +436,10 @@ struct smip_job {
     dptr<double> cons, scratch;
     uint64_t nframes = 0;
     bool finalized = false;
+    // B64 has been divided by nframes: set by the first finalisation attempt, so a
+    // retry after a failed phase recovery does not divide again, and no frame can
+    // be added to a divided sum
+    bool scaled = false;
     uint64_t reco_calls = 0;
     RecoWork* reco = nullptr; // phase-recovery scratch, made with the job
     std::unique_ptr<smip_bispec> view;
@@ -671,6 +679,10 @@ extern "C" SMIP_API int smip_ctx_destroy(smip_ctx* ctx)
     return guarded([&] {
         if (!ctx)
             return;
+        const int64_t live = ctx->live.load();
+        if (live > 0)
+            fail(SMIP_ERR_LOGIC, "context still has " + std::to_string(live)
+                    + " live objects (bispectra, frame stacks, cross-correlations, jobs or servers); destroy them first");
         destroy_stream(ctx->stream);
         cache_trim(ctx->device);
         delete ctx;
@@ -709,6 +721,7 @@ extern "C" SMIP_API int smip_bispec_create(smip_ctx* ctx, const uint64_t dims[4]
             "bispectrum dtype must be complex64 or complex128");
         ensure_device(ctx);
         auto b = std::make_unique<smip_bispec>();
+        b->ctx_ref.hold(ctx);
         b->ctx = ctx;
         b->dtype = dtype;
         if (dims[0] || dims[1] || dims[2] || dims[3])
@@ -1109,6 +1122,7 @@ extern "C" SMIP_API int smip_xcorr_create(smip_ctx* ctx, const double* ref, uint
         require(ctx && ref && out, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
         ensure_device(ctx);
         auto x = std::make_unique<smip_xcorr>();
+        x->ctx_ref.hold(ctx);
         x->ctx = ctx;
         x->X = (int)ncols;
         x->Y = (int)nrows;
@@ -1224,6 +1238,7 @@ extern "C" SMIP_API int smip_frames_create(smip_ctx* ctx, int32_t dtype, uint64_
         require(dtype >= SMIP_DTYPE_U8 && dtype <= SMIP_DTYPE_F64, SMIP_ERR_INVALID_ARGUMENT, "bad frame dtype");
         ensure_device(ctx);
         auto f = std::make_unique<smip_frames>();
+        f->ctx_ref.hold(ctx);
         f->ctx = ctx;
         f->dtype = dtype;
         f->X = ncols;
@@ -1279,8 +1294,10 @@ JobSizes job_sizes(const smip_job_params& p)
     require(pow2(p.ncols) && pow2(p.nrows), SMIP_ERR_INVALID_ARGUMENT,
         "job frame extents must be powers of two in [8, 2048]");
     // the accumulate loops clip v to the frame's range (include/bispectrum.h:383-391), so
-    // b may exceed N; phase recovery bounds the candidate window at b/2 <= 63
-    require(p.bdepth >= 1 && p.bdepth <= 127, SMIP_ERR_INVALID_ARGUMENT, "bispectrum depth must satisfy 1 <= b <= 127");
+    // b may exceed N; phase recovery evaluates a visit's (2h+1)^2 candidates in one
+    // workgroup, at most 65536 of them: sizes(b) = 2(b/2)+1 <= 255
+    require(p.bdepth >= 1 && p.bdepth <= 255, SMIP_ERR_INVALID_ARGUMENT,
+        "bispectrum depth must satisfy 1 <= b <= 255 (phase recovery takes (2(b/2)+1)^2 <= 65536 candidates per visit)");
     require(p.frame_dtype == SMIP_DTYPE_U8 || p.frame_dtype == SMIP_DTYPE_U16 || p.frame_dtype == SMIP_DTYPE_F32,
         SMIP_ERR_INVALID_ARGUMENT, "frame dtype must be u8, u16 or f32");
     require(p.precision == SMIP_PRECISION_FP64 || p.precision == SMIP_PRECISION_MIXED, SMIP_ERR_INVALID_ARGUMENT,
@@ -1566,6 +1583,25 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
     }
 }
 
+// B /= complex(n, 0) (smip-cli/main.cpp:312), once per job whatever happens after it
+void scale_once(smip_job* j, hipStream_t s)
+{
+    if (j->scaled)
+        return;
+    launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, s);
+    j->scaled = true;
+}
+
+// Failure injection for tests of the error paths: SMIP_INJECT_FAIL=<stage> makes
+// that stage fail with SMIP_ERR_OUT_OF_RANGE (the code of a failing get_element)
+// before it launches anything.  Read on every call, so a test can clear it.
+void injected_failure(const char* stage)
+{
+    const char* e = std::getenv("SMIP_INJECT_FAIL");
+    if (e && std::strcmp(e, stage) == 0)
+        fail(SMIP_ERR_OUT_OF_RANGE, std::string("injected failure of stage ") + stage);
+}
+
 void grow_shifts(smip_job* j, uint64_t total)
 {
     if (!j->p.saa || total <= j->shifts_cap)
@@ -1617,6 +1653,7 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
         if (z.bytes > fr)
             fail(SMIP_ERR_NO_MEMORY, "job footprint " + std::to_string(z.bytes) + " B exceeds free device memory");
         auto j = std::make_unique<smip_job>();
+        j->ctx_ref.hold(ctx);
         j->ctx = ctx;
         j->p = *p;
         j->d = z.d;
@@ -1694,7 +1731,7 @@ extern "C" SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, u
 {
     return guarded([&] {
         require(j && f, SMIP_ERR_INVALID_ARGUMENT, "null pointer");
-        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(!j->finalized && !j->scaled, SMIP_ERR_LOGIC, "job already finalized");
         require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
         require(f->dtype == j->p.frame_dtype && f->X == j->p.ncols && f->Y == j->p.nrows, SMIP_ERR_INVALID_ARGUMENT,
             "frame stack does not match the job");
@@ -1849,7 +1886,7 @@ extern "C" SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames
 {
     return guarded([&] {
         require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
-        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(!j->finalized && !j->scaled, SMIP_ERR_LOGIC, "job already finalized");
         require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
         ensure_device(j->ctx);
         grow_shifts(j, j->nframes + count);
@@ -1872,7 +1909,7 @@ extern "C" SMIP_API int smip_job_push_frames_host(smip_job* j, const void* frame
 {
     return guarded([&] {
         require(j && (frames || count == 0), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
-        require(!j->finalized, SMIP_ERR_LOGIC, "job already finalized");
+        require(!j->finalized && !j->scaled, SMIP_ERR_LOGIC, "job already finalized");
         ensure_device(j->ctx);
         ensure_ingest(j);
         const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
@@ -1901,10 +1938,10 @@ extern "C" SMIP_API int smip_job_finalize(smip_job* j)
         require(j->nframes > 0, SMIP_ERR_RUNTIME, "Array_base<T>::operator/=(T) : division by zero");
         hipStream_t s = j->stream;
         const smip_job_params& p = j->p;
-        // bispectrum /= complex(n, 0)
-        launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, s);
-        hipEvent_t e0 = mark(j);
         check_reco_geometry(j->d, p.ncols, p.nrows);
+        scale_once(j, s); // bispectrum /= complex(n, 0)
+        hipEvent_t e0 = mark(j);
+        injected_failure("reco");
         j->reco_calls = (uint64_t)run_reconstruct(j->reco, j->B64.get(), 0, j->phases.get(), j->flags.get(),
             j->cons.get(), s);
         hipEvent_t e1 = mark(j);
@@ -1985,8 +2022,7 @@ int smip_amd::job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* s
             for (uint64_t k = 0; k < n; ++k) {
                 smip_job* j = jobs[k];
                 flush_staged(j); // the last, partial batch of pushed frames
-                // bispectrum /= complex(n, 0)
-                launch_scale_c64(j->B64.get(), j->d.base_size, (double)j->nframes, 0.0, SMIP_OP_DIV, j->stream);
+                scale_once(j, j->stream); // bispectrum /= complex(n, 0)
                 hipEvent_t e = event(false);
                 SMIP_HIP(hipEventRecord(e, j->stream));
                 SMIP_HIP(hipStreamWaitEvent(cs, e, 0));
@@ -2006,6 +2042,7 @@ int smip_amd::job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* s
                 co[k] = j->cons.get();
             }
             hipEvent_t t0 = event(true), t1 = event(true);
+            injected_failure("reco");
             SMIP_HIP(hipEventRecord(t0, cs));
             run_reconstruct_many(work.data(), (int)n, B.data(), f32.data(), ph.data(), fl.data(), co.data(),
                 calls.data(), st.data(), cs);

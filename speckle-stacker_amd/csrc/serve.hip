@@ -55,6 +55,7 @@ double ms_between(Clock::time_point a, Clock::time_point b)
 } // namespace
 
 struct smip_server {
+    CtxRef ctx_ref;
     smip_ctx* ctx = nullptr;
     uint64_t budget = 0;
     int max_active = 16;
@@ -69,9 +70,13 @@ struct smip_server {
     uint64_t n_submitted = 0, n_admitted = 0, n_completed = 0, n_failed = 0, frames = 0;
     uint64_t n_groups = 0, n_grouped = 0; // finalisation groups and the jobs they held
     std::vector<std::thread> workers;
-    // one stream per worker with a hardware queue of its own, for the phase
-    // recoveries of the groups it finalises: on a queue shared with the jobs'
-    // streams, a recovery's chain of launches would hold up their batches
+    // streams with hardware queues of their own for the phase recoveries of the
+    // groups the workers finalise (worker w uses qstreams[w % size]): on a queue
+    // shared with the jobs' streams, a recovery's chain of launches would hold up
+    // their batches.  At most kQueueStreams of them: every such stream takes one
+    // of the device's hardware queue slots for the server's lifetime, and beyond
+    // the slots the command processor time-slices the queues.
+    static constexpr int kQueueStreams = 8;
     std::vector<hipStream_t> qstreams;
 
     // first-fit admission in arrival order; caller holds m
@@ -184,7 +189,8 @@ struct smip_server {
                 std::vector<smip_job*> handles;
                 for (ServedJob* g : group)
                     handles.push_back(g->job);
-                rc = smip_amd::job_finalize_many_on(handles.data(), handles.size(), gst.data(), qstreams[w]);
+                rc = smip_amd::job_finalize_many_on(handles.data(), handles.size(), gst.data(),
+                    qstreams[(size_t)w % qstreams.size()]);
                 if (rc != SMIP_OK && std::all_of(gst.begin(), gst.end(), [](int32_t c) { return c == SMIP_OK; }))
                     std::fill(gst.begin(), gst.end(), rc); // the group as a whole failed
             } else {
@@ -245,6 +251,7 @@ extern "C" SMIP_API int smip_server_create(smip_ctx* ctx, const smip_server_para
         if (p)
             q = *p;
         auto s = std::make_unique<smip_server>();
+        s->ctx_ref.hold(ctx);
         s->ctx = ctx;
         uint64_t free_b = 0, total_b = 0;
         if (smip_ctx_mem_info(ctx, &free_b, &total_b) != SMIP_OK)
@@ -255,7 +262,7 @@ extern "C" SMIP_API int smip_server_create(smip_ctx* ctx, const smip_server_para
         if (nw > 64 || s->max_active > 4096)
             fail(SMIP_ERR_INVALID_ARGUMENT, "server parameters out of range");
         smip_amd::ensure_device(ctx);
-        for (int w = 0; w < nw; ++w)
+        for (int w = 0; w < std::min(nw, smip_server::kQueueStreams); ++w)
             s->qstreams.push_back(smip_amd::make_queue_stream());
         smip_server* raw = s.get();
         for (int w = 0; w < nw; ++w)

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -312,7 +313,34 @@ uint64_t count_triples(const LoopBounds& L);
 struct smip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // objects made on this context and not yet destroyed (bispectra, frame
+    // stacks, cross-correlations, jobs, servers); smip_ctx_destroy refuses while
+    // any is alive, since each of them reads ctx when it is destroyed
+    std::atomic<int64_t> live { 0 };
 };
+
+namespace smip_amd {
+// Member of every object made on a context: counts it in ctx->live for its lifetime.
+struct CtxRef {
+    smip_ctx* ctx = nullptr;
+    CtxRef() = default;
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    void hold(smip_ctx* c)
+    {
+        if (ctx)
+            ctx->live.fetch_sub(1);
+        ctx = c;
+        if (ctx)
+            ctx->live.fetch_add(1);
+    }
+    ~CtxRef()
+    {
+        if (ctx)
+            ctx->live.fetch_sub(1);
+    }
+};
+} // namespace smip_amd
 
 namespace smip_amd {
 inline void ensure_device(smip_ctx* ctx) { SMIP_HIP(hipSetDevice(ctx->device)); }

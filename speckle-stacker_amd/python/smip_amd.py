@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -189,16 +190,27 @@ def _check(rc: int) -> None:
 
 # --------------------------------------------------------------------------
 class Context:
-    """One GPU (smip_ctx)."""
+    """One GPU (smip_ctx).
+
+    The library refuses to destroy a context while objects made on it are
+    alive, so close() first closes this context's live servers, then its jobs,
+    then its other objects."""
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()
         _check(lib().smip_ctx_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
 
     def close(self):
         if self.h:
-            lib().smip_ctx_destroy(self.h)
+            order = {Server: 0, Job: 1}
+            for obj in sorted(list(self._children), key=lambda o: order.get(type(o), 2)):
+                obj.close()
+            _check(lib().smip_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -232,6 +244,7 @@ class Bispectrum:
             _check(lib().smip_bispec_create(ctx.h, d, code, C.byref(h)))
             self.h = h
             self.borrowed = False
+            ctx._adopt(self)
 
     def close(self):
         if self.h and not self.borrowed:
@@ -347,6 +360,7 @@ class CrossCorrelation:
         _check(lib().smip_xcorr_create(ctx.h, _p(a), a.shape[1], a.shape[0], C.byref(h)))
         self.h = h
         self.shape = a.shape
+        ctx._adopt(self)
 
     def close(self):
         if self.h:
@@ -391,6 +405,7 @@ class Frames:
         h = C.c_void_p()
         _check(lib().smip_frames_create(ctx.h, _NP_DTYPES[np.dtype(dtype)], ncols, nrows, n, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         self.shape = tuple(shape)
         self.dtype = np.dtype(dtype)
         if frames is not None:
@@ -472,6 +487,7 @@ class Job:
             h = C.c_void_p()
             _check(lib().smip_job_create(ctx.h, C.byref(params), C.byref(h)))
             _handle = h
+            ctx._adopt(self)
         self.h = _handle
         self.ctx = ctx
         self.p = params
@@ -565,6 +581,7 @@ class Server:
         _check(lib().smip_server_create(ctx.h, C.byref(prm), C.byref(h)))
         self.h = h
         self.ctx = ctx
+        ctx._adopt(self)
         self.params: dict[int, JobParams] = {}
 
     def close(self):

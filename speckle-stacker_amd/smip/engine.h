@@ -39,19 +39,19 @@ inline void check(int rc)
 }
 
 // Device context shared by every drop-in object (device from SMIP_DEVICE, default 0).
+// It is made on first use and never destroyed: a static destructor that released
+// it would call HIP while the process exits, after or beside the HIP runtime's
+// own teardown and while static drop-in objects of other translation units may
+// still hold it.  The runtime reclaims the context's stream at exit.
 inline smip_ctx* context()
 {
-    struct Holder {
-        smip_ctx* ctx = nullptr;
-        Holder()
-        {
-            const char* dev = std::getenv("SMIP_DEVICE");
-            check(smip_ctx_create(dev ? std::atoi(dev) : 0, &ctx));
-        }
-        ~Holder() { smip_ctx_destroy(ctx); }
-    };
-    static Holder h;
-    return h.ctx;
+    static smip_ctx* const ctx = [] {
+        const char* dev = std::getenv("SMIP_DEVICE");
+        smip_ctx* c = nullptr;
+        check(smip_ctx_create(dev ? std::atoi(dev) : 0, &c));
+        return c;
+    }();
+    return ctx;
 }
 
 } // namespace smip::detail

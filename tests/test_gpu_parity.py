@@ -299,3 +299,57 @@ def test_assign_converts_element_type(ctx):
     back = S.Bispectrum(ctx, (16, 16, 8, 8), np.complex128)
     S._check(lib.smip_bispec_assign(back.h, f.h))
     assert np.array_equal(back.download(), vals.astype(np.complex64).astype(np.complex128))
+
+
+def _frames_rect(n, ncols, nrows, dtype=np.uint8, seed=1, hi=255):
+    """_frames on an nrows x ncols grid (blob pair plus noise, a unique correlation peak)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:nrows, 0:ncols]
+    out = np.empty((n, nrows, ncols), dtype=dtype)
+    for f in range(n):
+        cx = rng.integers(ncols // 4, 3 * ncols // 4)
+        cy = rng.integers(nrows // 4, 3 * nrows // 4)
+        img = 0.6 * hi * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / 8.0)
+        img += 0.35 * hi * np.exp(-((x - cx - 5) ** 2 + (y - cy - 3) ** 2) / 6.0)
+        img += rng.uniform(0, 0.05 * hi, size=(nrows, ncols))
+        out[f] = np.clip(np.rint(img), 0, hi).astype(dtype) if np.issubdtype(dtype, np.integer) else img.astype(dtype)
+    return out
+
+
+@pytest.mark.parametrize("ncols,nrows", [(64, 128), (128, 32)])
+@pytest.mark.parametrize("precision", [S.PRECISION_MIXED, S.PRECISION_FP64])
+def test_engine_non_square(ctx, ncols, nrows, precision):
+    """Frames with ncols != nrows through the whole job (row and column transforms,
+    the half-plane layout, the tile items, the phase map and the synthesis), so an
+    x/y mix-up anywhere shows: shifts, sum image and flags bit-exact, bispectrum
+    and image to the engine's tolerances."""
+    b, p, n = 16, 24, 9
+    frames = _frames_rect(n, ncols, nrows, seed=ncols + nrows)
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=precision, frames_per_batch=4)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    mixed = precision == S.PRECISION_MIXED
+    assert _rel(got.bispectrum, want.bispectrum) < (2e-5 if mixed else 1e-11)
+    assert _rel(got.powerspec, want.powerspec) < (1e-5 if mixed else 1e-11)
+    assert _rel_l2(got.image, want.image) < (1e-5 if mixed else 1e-9)
+
+
+@pytest.mark.parametrize("ncols,nrows", [(128, 64), (32, 64)])
+def test_xcorr_non_square(ctx, ncols, nrows):
+    """CrossCorrelation<double> on ncols != nrows: exact integer shifts and the
+    correlation array against the oracle."""
+    rng = np.random.default_rng(ncols * 3 + nrows)
+    ref = _frames_rect(1, ncols, nrows, np.float64, seed=5)[0]
+    xc = S.CrossCorrelation(ctx, ref)
+    for _ in range(8):
+        dx = int(rng.integers(-ncols // 3, ncols // 3 + 1))
+        dy = int(rng.integers(-nrows // 3, nrows // 3 + 1))
+        fr = np.roll(ref, (dy, dx), axis=(0, 1))
+        want = O.displacement(O.xcorr(ref, fr))
+        assert xc(fr) == want == (dx, dy)
+    c = xc.get_correlation_array()
+    assert c.shape == (nrows, ncols)
+    assert _rel(c, O.xcorr(ref, fr)) < 1e-12
