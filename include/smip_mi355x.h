@@ -109,6 +109,17 @@ SMIP_API int smip_bispec_accumulate_from_fft(smip_bispec* b, const double* fft, 
 /* get_elements(idx_list, results), include/bispectrum.h:514-523 (get_element
  * :482-512 for n == 1): indices are n x 4 ints, values n complex doubles. */
 SMIP_API int smip_bispec_get_elements(const smip_bispec* b, const int32_t* indices, uint64_t n, double* values);
+/* smip_bispec_get_elements without any device or pinned allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * between allocation-free sections of a serving loop, harnesses that forbid
+ * hipMalloc).  The lookups, values and statuses are staged in a fixed scratch
+ * array the library holds on each GPU for the life of the process (about 2.3 MB,
+ * its address resolved when the first context is made on that GPU).  At most
+ * 65536 lookups per call; more -> SMIP_ERR_INVALID_ARGUMENT.  Calls on one GPU
+ * share that array, so the library runs them one at a time.  Same results and
+ * errors as smip_bispec_get_elements. */
+SMIP_API int smip_bispec_get_elements_noalloc(const smip_bispec* b, const int32_t* indices, uint64_t n,
+    double* values);
 /* put_element, include/bispectrum.h:525-535 */
 SMIP_API int smip_bispec_put_element(smip_bispec* b, const int32_t idx[4], const double value[2]);
 /* calc_offset (no range check, modulo 2^64), include/bispectrum.h:318-349 */

@@ -645,6 +645,81 @@ __global__ void k_frame_to_f64(const void* __restrict__ in, int dtype, int64_t n
 } // namespace
 
 // ---------------------------------------------------------------------------
+// Scratch of smip_bispec_get_elements_noalloc: one copy of this array per GPU,
+// held until the process ends.  Laid out as the staged lookups (4 x int32 each),
+// their values (double2), their statuses (int32) and the first failure
+// (int32 index, int32 code).  Host code must not pass `scratch` itself to a
+// kernel or a copy (in host code the name is a stand-in, not a device address):
+// it uses the address hipGetSymbolAddress returned for the current device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint64_t kNoAllocLookups = (uint64_t)1 << 16;
+constexpr size_t kNaIdx = 0;
+constexpr size_t kNaOut = kNaIdx + sizeof(int32_t) * 4 * kNoAllocLookups;
+constexpr size_t kNaStatus = kNaOut + sizeof(double2) * kNoAllocLookups;
+constexpr size_t kNaFirst = kNaStatus + sizeof(int32_t) * kNoAllocLookups;
+constexpr size_t kNaBytes = kNaFirst + 256;
+static_assert(kNaOut % 256 == 0 && kNaStatus % 256 == 0 && kNaFirst % 256 == 0, "scratch parts 256-byte aligned");
+} // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch`, per device
+std::mutex g_scratch_use[kMaxDevices];        // one get_elements_noalloc call per device at a time
+
+// Resolves this device's copy of `scratch` (the device must be current); called
+// by smip_ctx_create, once per device.
+void noalloc_scratch_setup(int device)
+{
+    require(device >= 0 && device < kMaxDevices, SMIP_ERR_INVALID_ARGUMENT, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device])
+        return;
+    void* ptr = nullptr;
+    SMIP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, SMIP_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+}
+
+uint8_t* noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    uint8_t* p = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+    require(p != nullptr, SMIP_ERR_LOGIC, "no scratch address for this device (no context was created on it)");
+    return p;
+}
+
+// first[0] = lowest index i < n with status[i] != 0 (n if none), first[1] = its
+// status; one workgroup.
+__global__ __launch_bounds__(1024) void k_first_failure(const int32_t* __restrict__ status, int64_t n,
+    int32_t* __restrict__ first)
+{
+    __shared__ int32_t lo[16];
+    int32_t m = (int32_t)n;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        if (status[i] != 0) {
+            m = (int32_t)i;
+            break; // later entries of this thread's stride have larger indices
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        m = min(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0)
+        lo[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+            m = min(m, lo[k]);
+        first[0] = m;
+        first[1] = m < n ? status[m] : 0;
+    }
+}
+} // namespace
+
+// ---------------------------------------------------------------------------
 // misc
 // ---------------------------------------------------------------------------
 extern "C" SMIP_API const char* smip_last_error(void) { return last_error_text(); }
@@ -669,6 +744,7 @@ extern "C" SMIP_API int smip_ctx_create(int32_t device, smip_ctx** out)
         auto c = std::make_unique<smip_ctx>();
         c->device = device;
         SMIP_HIP(hipSetDevice(device));
+        noalloc_scratch_setup(device);
         c->stream = make_stream();
         *out = c.release();
     });
@@ -824,6 +900,46 @@ extern "C" SMIP_API int smip_bispec_get_elements(const smip_bispec* b, const int
                 fail(hst[i], "Bispectrum: element access out of bounds: indices = [" + std::to_string(q[0]) + ", "
                         + std::to_string(q[1]) + ", " + std::to_string(q[2]) + ", " + std::to_string(q[3]) + "]");
             }
+    });
+}
+
+extern "C" SMIP_API int smip_bispec_get_elements_noalloc(const smip_bispec* b, const int32_t* indices, uint64_t n,
+    double* values)
+{
+    return guarded([&] {
+        require(b && (n == 0 || (indices && values)), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (n > kNoAllocLookups)
+            fail(SMIP_ERR_INVALID_ARGUMENT, "smip_bispec_get_elements_noalloc takes at most "
+                    + std::to_string(kNoAllocLookups) + " lookups per call, got " + std::to_string(n));
+        if (n == 0)
+            return;
+        if (b->d.base_size == 0)
+            fail(SMIP_ERR_OUT_OF_RANGE, "Bispectrum: element address out of bounds.");
+        ensure_device(b->ctx);
+        const int dev = b->ctx->device;
+        uint8_t* base = noalloc_scratch(dev);
+        int32_t* idx = reinterpret_cast<int32_t*>(base + kNaIdx);
+        double2* out = reinterpret_cast<double2*>(base + kNaOut);
+        int32_t* st = reinterpret_cast<int32_t*>(base + kNaStatus);
+        int32_t* first = reinterpret_cast<int32_t*>(base + kNaFirst);
+        hipStream_t s = b->ctx->stream;
+        int32_t bad[2] = { 0, 0 };
+        {
+            // calls on one device share its scratch: one at a time
+            std::lock_guard<std::mutex> use(g_scratch_use[dev]);
+            SMIP_HIP(hipMemcpyAsync(idx, indices, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, s));
+            launch_get_elements(b->d, b->data, b->dtype == SMIP_DTYPE_C64, idx, (int64_t)n, out, st, s);
+            hipLaunchKernelGGL(k_first_failure, dim3(1), dim3(1024), 0, s, st, (int64_t)n, first);
+            SMIP_HIP(hipGetLastError());
+            SMIP_HIP(hipMemcpyAsync(values, out, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+            SMIP_HIP(hipMemcpyAsync(bad, first, sizeof(bad), hipMemcpyDeviceToHost, s));
+            SMIP_HIP(hipStreamSynchronize(s));
+        }
+        if ((uint64_t)bad[0] < n) {
+            const int32_t* q = indices + 4 * (uint64_t)bad[0];
+            fail(bad[1], "Bispectrum: element access out of bounds: indices = [" + std::to_string(q[0]) + ", "
+                    + std::to_string(q[1]) + ", " + std::to_string(q[2]) + ", " + std::to_string(q[3]) + "]");
+        }
     });
 }
 

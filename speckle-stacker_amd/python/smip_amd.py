@@ -103,6 +103,7 @@ _SIGS = {
     "smip_bispec_info_get": (C.c_int, [_P, _P]),
     "smip_bispec_accumulate_from_fft": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
     "smip_bispec_get_elements": (C.c_int, [_P, _P, C.c_uint64, _P]),
+    "smip_bispec_get_elements_noalloc": (C.c_int, [_P, _P, C.c_uint64, _P]),
     "smip_bispec_put_element": (C.c_int, [_P, _P, _P]),
     "smip_bispec_calc_offset": (C.c_uint64, [_P, _P]),
     "smip_bispec_calc_indices": (C.c_int, [_P, C.c_uint64, _P]),
