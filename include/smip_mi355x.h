@@ -196,7 +196,7 @@ SMIP_API int smip_frames_upload(smip_frames* f, uint64_t first, uint64_t count, 
 typedef struct smip_job smip_job;
 
 typedef struct smip_job_params {
-    uint64_t ncols, nrows;     /* frame extents (powers of two, 8..4096) */
+    uint64_t ncols, nrows;     /* frame extents, any value in [2, 2048] */
     uint64_t bdepth;           /* -b, bispectrum extent of dims 3 and 4 */
     double reco_radius;        /* -p */
     int32_t frame_dtype;       /* SMIP_DTYPE_U8 / U16 / F32 */

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", f"-I{INCLUDE}",
 ]
-HIP_SOURCES = ["fft.hip", "bispec.hip", "reco.hip", "post.hip", "abi.hip", "serve.hip", "tune.hip"]
+HIP_SOURCES = ["fft.hip", "fft_mixed.hip", "bispec.hip", "reco.hip", "post.hip", "abi.hip", "serve.hip", "tune.hip"]
 
 
 def _run(cmd: list[str]) -> None:

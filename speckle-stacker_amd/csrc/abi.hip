@@ -1395,6 +1395,16 @@ int saa_chunk(uint64_t ncols, uint64_t nrows, int batch)
     return std::min(batch, (int)c);
 }
 
+// Bytes of per-workgroup correlation maxima the shift-and-add writes per frame:
+// nrows / 2 entries on the power-of-two path (two rows per transform), at most
+// nrows (one row per transform) on the other.
+uint64_t saa_best_bytes(uint64_t ncols, uint64_t nrows)
+{
+    auto pow2 = [](uint64_t n) { return n >= 8 && (n & (n - 1)) == 0; };
+    const uint64_t entries = pow2(ncols) && pow2(nrows) ? nrows / 2 : nrows;
+    return entries * sizeof(RowBest);
+}
+
 struct JobSizes {
     Desc d;
     LoopBounds L;
@@ -1406,9 +1416,9 @@ struct JobSizes {
 
 JobSizes job_sizes(const smip_job_params& p)
 {
-    auto pow2 = [](uint64_t n) { return n >= 8 && n <= 2048 && (n & (n - 1)) == 0; };
-    require(pow2(p.ncols) && pow2(p.nrows), SMIP_ERR_INVALID_ARGUMENT,
-        "job frame extents must be powers of two in [8, 2048]");
+    // any extents FFTW would take, up to the longest transform the LDS holds
+    require(p.ncols >= 2 && p.nrows >= 2 && p.ncols <= (uint64_t)kMaxFftLength && p.nrows <= (uint64_t)kMaxFftLength,
+        SMIP_ERR_INVALID_ARGUMENT, "job frame extents must be in [2, 2048]");
     // the accumulate loops clip v to the frame's range (include/bispectrum.h:383-391), so
     // b may exceed N; phase recovery evaluates a visit's (2h+1)^2 candidates in one
     // workgroup, at most 65536 of them: sizes(b) = 2(b/2)+1 <= 255
@@ -1444,7 +1454,7 @@ JobSizes job_sizes(const smip_job_params& p)
         b += (uint64_t)z.batch * ((uint64_t)z.hp.fs * sizeof(double2) + hx * Y * sizeof(double2));
     if (p.saa)
         b += hx * Y * sizeof(double2)
-            + (uint64_t)saa_chunk(X, Y, z.batch) * (hx * Y * sizeof(double2) + Y * sizeof(double))
+            + (uint64_t)saa_chunk(X, Y, z.batch) * (hx * Y * sizeof(double2) + saa_best_bytes(X, Y))
             + np * sizeof(double) + (uint64_t)z.batch * 2 * sizeof(int32_t);
     b += 2 * (uint64_t)z.batch * np * dtype_size(p.frame_dtype);         // two host staging buffers
     b += np * (2 * sizeof(double2) + sizeof(uint8_t) + sizeof(double)); // outputs
@@ -1803,8 +1813,10 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
             j->ref_spec = dmake<double2>(hx * Y);
             const size_t sc = (size_t)saa_chunk(X, Y, j->batch);
             j->swork = dmake<double2>(sc * hx * Y);
-            // per-workgroup correlation maxima: at most nrows / 2 RowBest (16 B) per frame
-            j->corr = dmake<double>(sc * Y);
+            // per-workgroup correlation maxima (RowBest, 16 B each) of every frame of a pass
+            require((uint64_t)fft_shift_best_entries(j->plan) * sizeof(RowBest) <= saa_best_bytes(X, Y), SMIP_ERR_LOGIC,
+                "shift-and-add maxima exceed their buffer");
+            j->corr = dmake<double>(sc * saa_best_bytes(X, Y) / sizeof(double));
             j->sum = dmake<double>(np);
             SMIP_HIP(hipMemsetAsync(j->sum.get(), 0, sizeof(double) * np, s));
             j->bshift = dmake<int32_t>(2 * (size_t)j->batch);

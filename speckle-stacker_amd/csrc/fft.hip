@@ -16,18 +16,11 @@
 #include <cmath>
 #include <vector>
 
+#include "fft_plan.h"
 #include "smip_complex.h"
 #include "smip_internal.h"
 
 namespace smip_amd {
-
-struct FftPlan {
-    int32_t X, Y, lx, ly;
-    float2* twx32 = nullptr;
-    float2* twy32 = nullptr;
-    double2* twx64 = nullptr;
-    double2* twy64 = nullptr;
-};
 
 static int ilog2(int n)
 {
@@ -62,14 +55,17 @@ static V* upload_twiddles(int n, hipStream_t stream)
 
 FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream)
 {
+    if (ncols < 2 || nrows < 2 || ncols > kMaxFftLength || nrows > kMaxFftLength)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "frame extents must be in [2, 2048]");
     auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
-    if (!pow2(ncols) || !pow2(nrows) || ncols > 2048 || nrows > 2048)
-        fail(SMIP_ERR_INVALID_ARGUMENT, "frame extents must be powers of two in [8, 2048]");
     auto* p = new FftPlan;
     p->X = ncols;
     p->Y = nrows;
     p->lx = ilog2(ncols);
     p->ly = ilog2(nrows);
+    p->pow2 = pow2(ncols) && pow2(nrows);
+    p->mx = factorize(ncols);
+    p->my = factorize(nrows);
     try {
         p->twx32 = upload_twiddles<float2>(ncols, stream);
         p->twy32 = upload_twiddles<float2>(nrows, stream);
@@ -856,8 +852,12 @@ static void launch_power(const V* H, int64_t nf, const HalfPlane& hp, double* P,
 void launch_forward_f32(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
     const HalfPlane& hp, float2* out, float2* work, double* power_acc, hipStream_t s)
 {
-    launch_rows_fwd<float2>(p, frames, dtype, nf, absval, work, s);
-    launch_cols_fwd<float2, 0, float2>(p, work, nf, out, hp, s);
+    if (!p->pow2) {
+        mixed_forward_f32(p, frames, dtype, nf, absval, hp, out, work, s);
+    } else {
+        launch_rows_fwd<float2>(p, frames, dtype, nf, absval, work, s);
+        launch_cols_fwd<float2, 0, float2>(p, work, nf, out, hp, s);
+    }
     if (power_acc)
         launch_power<float2>(out, nf, hp, power_acc, s);
 }
@@ -865,8 +865,12 @@ void launch_forward_f32(const FftPlan* p, const void* frames, int dtype, int64_t
 void launch_forward_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
     const HalfPlane& hp, double2* out, double2* work, double* power_acc, hipStream_t s)
 {
-    launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
-    launch_cols_fwd<double2, 0, double2>(p, work, nf, out, hp, s);
+    if (!p->pow2) {
+        mixed_forward_f64(p, frames, dtype, nf, absval, hp, out, work, s);
+    } else {
+        launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
+        launch_cols_fwd<double2, 0, double2>(p, work, nf, out, hp, s);
+    }
     if (power_acc)
         launch_power<double2>(out, nf, hp, power_acc, s);
 }
@@ -874,6 +878,10 @@ void launch_forward_f64(const FftPlan* p, const void* frames, int dtype, int64_t
 void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval, double2* out,
     double2* work, hipStream_t s)
 {
+    if (!p->pow2) {
+        mixed_rfft_f64(p, frames, dtype, nf, absval, out, work, s);
+        return;
+    }
     HalfPlane hp {};
     launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
     launch_cols_fwd<double2, 1, double2>(p, work, nf, out, hp, s);
@@ -882,6 +890,10 @@ void launch_rfft_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf
 void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
     const double2* ref_spec, RowBest* best, double2* work, int32_t* shifts, hipStream_t s)
 {
+    if (!p->pow2) {
+        mixed_shift_f64(p, frames, dtype, nf, absval, ref_spec, best, work, shifts, s);
+        return;
+    }
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
     const int cb = cols_per_wg<double2>(Y); // cb * Y * 16 <= kColLds, as k_xcorr_cols's registers assume
@@ -893,13 +905,27 @@ void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t n
     hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
         s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
     SMIP_HIP(hipGetLastError());
+    launch_best_to_shift(best, nblk, nf, X, Y, shifts, s);
+}
+
+void launch_best_to_shift(const RowBest* best, int nblk, int64_t nf, int X, int Y, int32_t* shifts, hipStream_t s)
+{
     hipLaunchKernelGGL(k_best_to_shift, dim3((unsigned)nf), dim3(64), 0, s, best, nblk, nf, X, Y, shifts);
     SMIP_HIP(hipGetLastError());
+}
+
+int fft_shift_best_entries(const FftPlan* p)
+{
+    return p->pow2 ? p->Y / 2 : mixed_shift_blocks(p);
 }
 
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf, double* corr,
     double2* work, hipStream_t s)
 {
+    if (!p->pow2) {
+        mixed_xcorr_f64(p, ref_spec, spec, nf, corr, work, s);
+        return;
+    }
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<double2>(Y);
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
@@ -914,6 +940,10 @@ void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* 
 void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipStream_t s)
 {
     (void)work;
+    if (!p->pow2) {
+        mixed_c2c_inverse_f64(p, data, s);
+        return;
+    }
     const int X = p->X, Y = p->Y;
     const int rpw = rows_per_wg<double2>(X, Y);
     hipLaunchKernelGGL(k_rows_c2c, dim3(Y / rpw), dim3(kThreads), 2 * sizeof(double2) * rpw * X, s, data, X, p->lx,

@@ -212,6 +212,8 @@ enum class FrameType : int { u8 = SMIP_DTYPE_U8, u16 = SMIP_DTYPE_U16, f32 = SMI
 size_t dtype_size(int dtype);
 
 struct FftPlan; // twiddle tables for one frame shape
+constexpr int32_t kMaxFftLength = 2048; // longest transform held in LDS
+// any extents in [2, kMaxFftLength]
 FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream);
 void fft_plan_destroy(FftPlan*);
 
@@ -237,6 +239,8 @@ struct RowBest {
 // nf * nrows * (ncols/2 + 1) complex values; best nf * nrows / 2 entries.
 void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval,
     const double2* ref_spec, RowBest* best, double2* work, int32_t* shifts, hipStream_t s);
+// RowBest entries launch_shift_f64 writes per frame
+int fft_shift_best_entries(const FftPlan* p);
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf,
     double* corr, double2* work, hipStream_t s);
 // unnormalised backward complex 2-D FFT, FP64, in place on nrows x ncols.

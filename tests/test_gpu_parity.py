@@ -353,3 +353,58 @@ def test_xcorr_non_square(ctx, ncols, nrows):
     c = xc.get_correlation_array()
     assert c.shape == (nrows, ncols)
     assert _rel(c, O.xcorr(ref, fr)) < 1e-12
+
+
+@pytest.mark.parametrize("ncols,nrows,b,p", [(100, 100, 16, 30), (96, 128, 16, 40), (120, 90, 20, 30),
+                                              (45, 27, 8, 10), (131, 67, 12, 20), (7, 9, 4, 3)])
+@pytest.mark.parametrize("precision", [S.PRECISION_MIXED, S.PRECISION_FP64])
+def test_engine_any_size(ctx, ncols, nrows, b, p, precision):
+    """Frame extents FFTW takes but a power-of-two FFT does not (smip-cli/main.cpp:256-259):
+    even, odd, prime and mixed extents through the mixed-radix transforms of the
+    whole job, against the oracle (whose FFT is a direct DFT at such lengths)."""
+    n = 7
+    frames = _frames_rect(n, ncols, nrows, seed=ncols * 7 + nrows)
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=precision, frames_per_batch=3)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    assert np.array_equal(got.bispectrum == 0, want.bispectrum == 0)
+    mixed = precision == S.PRECISION_MIXED
+    assert _rel(got.bispectrum, want.bispectrum) < (2e-5 if mixed else 1e-11)
+    assert _rel(got.powerspec, want.powerspec) < (1e-5 if mixed else 1e-11)
+    assert _rel_l2(got.image, want.image) < (1e-5 if mixed else 1e-9)
+
+
+@pytest.mark.parametrize("ncols,nrows", [(2039, 6), (2048, 10), (6, 2039)])
+def test_longest_lengths_not_powers_of_two(ctx, ncols, nrows):
+    """The largest prime length under the 2048 limit (one 2039-point stage of
+    per-output sums) and the 2048-point mixed-radix transform beside a short
+    non-power-of-two side, in rows and in columns."""
+    frames = _frames_rect(3, ncols, nrows, seed=ncols + 3 * nrows)
+    b, p = 2, 2
+    want = O.pipeline(frames.astype(np.float64), b, p)
+    got = S.run_pipeline(ctx, frames, b, p, precision=S.PRECISION_MIXED)
+    assert np.array_equal(got.shifts, want.shifts)
+    assert np.array_equal(got.sum_image, want.sum_image)
+    assert np.array_equal(got.flags, want.flags)
+    assert _rel(got.bispectrum, want.bispectrum) < 2e-5
+    assert _rel(got.powerspec, want.powerspec) < 1e-5
+    assert _rel_l2(got.image, want.image) < 1e-5
+
+
+@pytest.mark.parametrize("ncols,nrows", [(100, 60), (131, 67), (9, 7)])
+def test_xcorr_any_size(ctx, ncols, nrows):
+    """CrossCorrelation<double> at extents that are not powers of two: exact shifts
+    and the correlation array against the oracle."""
+    rng = np.random.default_rng(ncols + nrows)
+    ref = _frames_rect(1, ncols, nrows, np.float64, seed=11)[0]
+    xc = S.CrossCorrelation(ctx, ref)
+    for _ in range(6):
+        dx = int(rng.integers(-(ncols // 3), ncols // 3 + 1))
+        dy = int(rng.integers(-(nrows // 3), nrows // 3 + 1))
+        fr = np.roll(ref, (dy, dx), axis=(0, 1))
+        want = O.displacement(O.xcorr(ref, fr))
+        assert xc(fr) == want
+    c = xc.get_correlation_array()
+    assert _rel(c, O.xcorr(ref, fr)) < 1e-12
