@@ -59,8 +59,17 @@ def test_job_footprint_counts_the_bispectrum():
     fp = S.job_footprint(p)
     assert fp > 40_612_425 * 16  # the FP64 fold buffer alone
     assert fp < 4 * 2 ** 30
+    # any side FFTW takes, up to the 2048 the engine's transforms hold in LDS
+    assert S.job_footprint(S.job_params(100, 90, 8, 8, np.uint8)) > 0
+    assert S.job_footprint(S.job_params(2, 2039, 2, 2, np.uint8)) > 0
+    for ncols, nrows in ((1, 64), (64, 2049), (4096, 4096)):
+        with pytest.raises(S.SmipError) as e:
+            S.job_footprint(S.job_params(ncols, nrows, 8, 8, np.uint8))
+        assert e.value.code == S.ERR_INVALID_ARGUMENT
+    # bispectrum depth up to 255: phase recovery's (2(b/2)+1)^2 <= 65536 candidates
+    assert S.job_footprint(S.job_params(64, 64, 255, 8, np.uint8)) > 0
     with pytest.raises(S.SmipError) as e:
-        S.job_footprint(S.job_params(100, 100, 8, 8, np.uint8))
+        S.job_footprint(S.job_params(64, 64, 256, 8, np.uint8))
     assert e.value.code == S.ERR_INVALID_ARGUMENT
 
 
