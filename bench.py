@@ -168,21 +168,27 @@ def workload_leg(ctx, name: str, nframes: int, passes: int, latency_runs: int) -
             "stages_ms": {k: round(stages[k], 2) for k in ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}}
 
 
-def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int, grouped: bool = False):
-    """njobs complete reconstructions submitted together; each thread drives one
-    job.  Each job finalises on its own, or (grouped) hands its accumulated
-    bispectrum to one finalizer thread that recovers the phases of every job
-    ready at that moment in one group (smip_job_finalize_many), as the serving
-    loop does."""
+def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int, grouped: bool = False,
+                     stagger_ms: float = 0.0):
+    """njobs complete reconstructions; job i arrives i * stagger_ms after the
+    first and one thread drives it.  Each job finalises on its own, or (grouped)
+    hands its accumulated bispectrum to one finalizer thread that recovers the
+    phases of every job ready at that moment in one group
+    (smip_job_finalize_many), as the serving loop does.  Latency is from a job's
+    own arrival to its completion."""
     lat = [0.0] * njobs
     errs = []
     stats = [None] * njobs
     ready = []
     cv = threading.Condition()
-    t_submit = time.perf_counter()
+    t_start = time.perf_counter()
+    t_arrive = [t_start + i * stagger_ms * 1e-3 for i in range(njobs)]
 
     def worker(i):
         try:
+            delay = t_arrive[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
             p = S.job_params(W["N"], W["N"], W["b"], W["p"], W["dtype"], S.PRECISION_MIXED, saa=True,
                              frames_per_batch=batch, vy_tile=vy)
             job = S.Job(ctx, p)
@@ -194,7 +200,7 @@ def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int, grouped: bool
                     cv.notify()
                 return
             job.finalize()
-            lat[i] = (time.perf_counter() - t_submit) * 1e3
+            lat[i] = (time.perf_counter() - t_arrive[i]) * 1e3
             stats[i] = job.stats()
             job.close()
         except Exception as e:  # reported, not hidden
@@ -217,9 +223,9 @@ def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int, grouped: bool
                     S.finalize_many(jobs)
             except Exception as e:
                 errs.append(repr(e))
-            t = (time.perf_counter() - t_submit) * 1e3
+            t = time.perf_counter()
             for i, j in group:
-                lat[i] = t
+                lat[i] = (t - t_arrive[i]) * 1e3
                 if j is not None:
                     stats[i] = j.stats()
                     j.close()
@@ -343,6 +349,8 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
     pct = lambda a, q: round(float(np.percentile(a, q)), 2)
     frames_done = int(sum(jobs[i]["n"] for i in done))
     return {"arrival": f"Poisson, {rate} jobs/s for {seconds} s", "jobs": len(jobs), "completed": len(done),
+            "window": f"{seconds} s of arrivals (the configured serving run is 600 s; the same rate, mix and "
+                      "camera model over a shorter window)",
             "failed": len(failures), "unfinished": len(jobs) - len(done) - len(failures),
             "failures": list(failures.values())[:3], "frames_of_completed_jobs": frames_done,
             "frames_pushed": int(frames_pushed), "wall_s": round(wall, 2),
@@ -500,56 +508,113 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
             "order": "longest first (frames x N^2 b^2)"}
 
 
-def precision_leg(ctx, n: int) -> dict:
+def phase_bins(m, d, u, fl) -> dict:
+    """Median |phase difference| of the mixed path and of the one-ulp FP64 run
+    against the FP64 path, over recovered points grouped by their FP64
+    consistency |mean phasor|: well-determined phases (consistency near 1)
+    against those the recursion barely constrains."""
+    c = d.consistency
+    out = {}
+    for lo, hi in ((0.5, 1.01), (0.1, 0.5), (0.01, 0.1), (0.0, 0.01)):
+        sel = fl & (c >= lo) & (c < hi)
+        if not sel.any():
+            continue
+        out[f"[{lo},{min(hi, 1.0)})"] = {"points": int(sel.sum()),
+                                         "mixed_median": float(np.median(np.abs(m.phases[sel] - d.phases[sel]))),
+                                         "fp64_one_ulp_median": float(np.median(np.abs(u.phases[sel] - d.phases[sel])))}
+    return out
+
+
+def precision_leg(ctx, n: int, batch: int = 256, chunk: int = 512) -> dict:
     """Error of the mixed path against the FP64 path on the 512x512 float32
-    extended-object stack (b = 64, p = 256), first n frames: bispectrum
-    magnitude and phase, phases and the image.  The FP64 GPU path is the
-    reference here (it matches the FP64 oracle to 1e-11 of max|B| on the
-    sizes the oracle finishes)."""
+    extended-object stack (b = 64, p = 256) at n frames: FP32 sums per batch of
+    `batch` frames folded in FP64, frames streamed from host memory in chunks of
+    `chunk` through smip_job_push_frames_host, as a camera would deliver them.
+    Reported: bispectrum magnitude and phase, phases, flags, shifts and the
+    image.  The FP64 GPU path is the reference here (it matches the FP64 oracle
+    to 1e-11 of max|B| on the sizes the oracle finishes, and its 3-frame result
+    at this size is checked against the oracle's in the GPU tests).  A third job
+    runs the FP64 path on the same frames moved by one float32 ulp: how far
+    rounding alone moves the result (the recursion's conditioning)."""
     W = S.WORKLOADS["extended512"]
-    frames = S.synth_frames(S.synth_params("extended512"), 0, n, threads=min(16, os.cpu_count() or 1))
-    res = {}
-    for prec in (S.PRECISION_MIXED, S.PRECISION_FP64):
-        res[prec] = S.run_pipeline(ctx, frames, W["b"], W["p"], precision=prec, with_bispectrum=True)
-        log(f"precision: {'mixed' if prec == S.PRECISION_MIXED else 'FP64'} run done")
-    m, d = res[S.PRECISION_MIXED], res[S.PRECISION_FP64]
+    N, b, pr = W["N"], W["b"], W["p"]
+    prm = S.synth_params("extended512")
+    jobs = {"mixed": S.Job(ctx, S.job_params(N, N, b, pr, np.float32, S.PRECISION_MIXED, frames_per_batch=batch)),
+            "fp64": S.Job(ctx, S.job_params(N, N, b, pr, np.float32, S.PRECISION_FP64)),
+            "fp64_ulp": S.Job(ctx, S.job_params(N, N, b, pr, np.float32, S.PRECISION_FP64))}
+    threads = min(16, os.cpu_count() or 1)
+    # the next chunk is generated while the GPU works on this one
+    gen_s = [0.0]
+
+    def gen(first):
+        t = time.perf_counter()
+        f = S.synth_frames(prm, first, min(chunk, n - first), threads=threads)
+        gen_s[0] += time.perf_counter() - t
+        return f
+
+    import concurrent.futures as cf
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=1) as ex:
+        nxt = ex.submit(gen, 0)
+        for first in range(0, n, chunk):
+            fr = nxt.result()
+            if first + chunk < n:
+                nxt = ex.submit(gen, first + chunk)
+            jobs["mixed"].push_frames_host(fr)
+            jobs["fp64"].push_frames_host(fr)
+            jobs["fp64_ulp"].push_frames_host(np.nextafter(fr, np.float32(np.inf)))
+            if (first // chunk) % 4 == 3:
+                log(f"precision: {first + len(fr)}/{n} frames pushed")
+    S.finalize_many(list(jobs.values()))
+    wall = time.perf_counter() - t0
+    log(f"precision: {n} frames through three jobs in {wall:.1f} s (generation {gen_s[0]:.1f} s)")
+    st = {k: j.stats() for k, j in jobs.items()}
+    res = {k: j.result(with_bispectrum=(k != "fp64_ulp")) for k, j in jobs.items()}
+    for j in jobs.values():
+        j.close()
+    m, d, u = res["mixed"], res["fp64"], res["fp64_ulp"]
     bm, bd = m.bispectrum, d.bispectrum
     amax = float(np.abs(bd).max())
     mag_num = mag_den = 0.0
     ph_w = ph_n = 0.0
     ph_max = 0.0
+    err_max = 0.0
     step = 1 << 24
     for k in range(0, bd.size, step):  # chunked: 4.5 GB per bispectrum
-        a, b = bm[k:k + step], bd[k:k + step]
-        aa, ab = np.abs(a), np.abs(b)
+        a, c = bm[k:k + step], bd[k:k + step]
+        aa, ab = np.abs(a), np.abs(c)
         mag_num += float(np.sum((aa - ab) ** 2))
         mag_den += float(np.sum(ab ** 2))
+        err_max = max(err_max, float(np.max(np.abs(a - c))))
         sig = ab > 1e-6 * amax
         if sig.any():
-            dph = np.abs(np.angle(a[sig] * np.conj(b[sig])))
+            dph = np.abs(np.angle(a[sig] * np.conj(c[sig])))
             ph_w += float(np.sum(dph * ab[sig]))
             ph_n += float(np.sum(ab[sig]))
             ph_max = max(ph_max, float(dph.max()))
     fl = m.flags.astype(bool) & d.flags.astype(bool)
-    log("precision: bispectrum errors done")
-    # conditioning: the FP64 path itself on the same frames moved by one float32 ulp
-    pert = np.nextafter(frames, np.float32(np.inf))
-    dp = S.run_pipeline(ctx, pert, W["b"], W["p"], precision=S.PRECISION_FP64, with_bispectrum=False)
     cons = d.consistency[d.flags.astype(bool)]
-    return {"workload": f"extended512: first {n} of 10000 frames 512x512 float32, b={W['b']}, p={W['p']}",
-            "fp64_vs_fp64_one_ulp_input": {
-                "image_rel_l2": float(np.linalg.norm(dp.image - d.image) / np.linalg.norm(d.image)),
-                "phase_max_abs_diff": float(np.abs(dp.phases[fl] - d.phases[fl]).max()),
-                "flags_equal": bool(np.array_equal(dp.flags, d.flags))},
-            "recovered_points": int(cons.size),
-            "consistency_below_1e-2": float(np.mean(cons < 1e-2)),
+    rel_img = lambda a, c: float(np.linalg.norm(np.abs(a) - np.abs(c)) / np.linalg.norm(np.abs(c)))
+    return {"workload": f"extended512: {n} of {W['n']} frames 512x512 float32, b={b}, p={pr}",
+            "frames_per_batch": batch, "ingest": f"smip_job_push_frames_host, chunks of {chunk} frames from "
+                                                 "pageable memory, generation overlapped with the GPU work",
             "reference": "FP64 GPU path",
+            "frames": int(st["mixed"]["frames"]), "fp64_folds": -(-n // batch),
+            "bispectrum_max_abs_err_over_max": err_max / amax,
             "bispectrum_magnitude_rel_l2": mag_num ** 0.5 / mag_den ** 0.5,
             "bispectrum_phase_rad": {"weighted_mean": ph_w / ph_n, "max_where_|B|>1e-6max": ph_max},
             "phase_max_abs_diff": float(np.abs(m.phases[fl] - d.phases[fl]).max()),
             "flags_equal": bool(np.array_equal(m.flags, d.flags)),
             "shifts_equal": bool(np.array_equal(m.shifts, d.shifts)),
-            "image_rel_l2": float(np.linalg.norm(m.image - d.image) / np.linalg.norm(d.image))}
+            "image_rel_l2": rel_img(m.image, d.image),
+            "fp64_vs_fp64_one_ulp_input": {"image_rel_l2": rel_img(u.image, d.image),
+                                           "phase_max_abs_diff": float(np.abs(u.phases[fl] - d.phases[fl]).max()),
+                                           "flags_equal": bool(np.array_equal(u.flags, d.flags))},
+            "phase_diff_by_consistency": phase_bins(m, d, u, fl),
+            "recovered_points": int(cons.size), "consistency_below_1e-2": float(np.mean(cons < 1e-2)),
+            "device_ms": {k: {q: round(v[q], 1) for q in ("accumulate_ms", "saa_ms", "reco_ms")}
+                          for k, v in st.items()},
+            "wall_s": round(wall, 1), "generation_s": round(gen_s[0], 1)}
 
 
 def main() -> int:
@@ -575,7 +640,7 @@ def main() -> int:
     ap.add_argument("--no-configs", action="store_true", help="skip the 128x128 and 512x512 legs")
     ap.add_argument("--only", default="",
                     help="comma list of legs to run: latency,configs,batch,precision,serve,cpu")
-    ap.add_argument("--precision-frames", type=int, default=128,
+    ap.add_argument("--precision-frames", type=int, default=10000,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
     if args.hw_queues:
@@ -718,31 +783,39 @@ def main() -> int:
                                         ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}}
         log(f"single job latency {out['single_job_latency_ms']}")
         log(f"running {args.jobs} concurrent complete jobs (accumulate + SAA + phases + synthesis)")
-        lat, stats = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile, grouped=True)
+        stagger = 40.0
+        lat, stats = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile, grouped=True,
+                                      stagger_ms=stagger)
         lat.sort()
         out["job_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2),
                                  "p99": round(float(np.percentile(lat, 99)), 2), "jobs": args.jobs,
-                                 "frames_per_job": n, "arrival": "all jobs submitted at t=0",
+                                 "each_ms": [round(x, 1) for x in lat],
+                                 "frames_per_job": n,
+                                 "arrival": f"one job every {stagger:.0f} ms, latency from its own arrival",
                                  "finalization": "grouped: the jobs whose frames are in recover their phases "
                                                  "together (smip_job_finalize_many), as the serving loop does"}
         s0 = stats[0]
         out["job_stage_ms"] = {k: round(s0[k], 2) for k in ("accumulate_ms", "saa_ms", "reco_ms", "synth_ms")}
         log(f"latency p50 {out['job_latency_ms']['p50']} ms p99 {out['job_latency_ms']['p99']} ms; "
             f"stages {out['job_stage_ms']}")
-        lat_i, _ = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile, grouped=False)
+        lat_i, _ = run_latency_jobs(ctx, dev, W, args.jobs, args.batch, args.vy_tile, grouped=False,
+                                    stagger_ms=stagger)
         out["job_latency_ms_individual_finalize"] = {"p50": round(float(np.percentile(lat_i, 50)), 2),
                                                      "p99": round(float(np.percentile(lat_i, 99)), 2)}
         log(f"each job finalising on its own: {out['job_latency_ms_individual_finalize']}")
-        # accuracy of the mixed path against the FP64 GPU path on the same frames
-        na = min(n, 200)
-        rm = S.run_pipeline(ctx, frames[:na], b, pr, precision=S.PRECISION_MIXED, with_bispectrum=False)
-        rd = S.run_pipeline(ctx, frames[:na], b, pr, precision=S.PRECISION_FP64, with_bispectrum=False)
+        # accuracy of the mixed path against the FP64 GPU path on the whole stack
+        rm = S.run_pipeline(ctx, frames, b, pr, precision=S.PRECISION_MIXED, with_bispectrum=True)
+        rd = S.run_pipeline(ctx, frames, b, pr, precision=S.PRECISION_FP64, with_bispectrum=True)
         out["mixed_vs_fp64_gpu"] = {
-            "frames": na,
+            "frames": n, "fp64_folds": -(-n // (args.batch or 512)),
+            "bispectrum_max_abs_err_over_max": float(np.max(np.abs(rm.bispectrum - rd.bispectrum))
+                                                     / np.max(np.abs(rd.bispectrum))),
             "image_rel_l2": float(np.linalg.norm(np.abs(rm.image) - np.abs(rd.image)) / np.linalg.norm(np.abs(rd.image))),
+            "phase_max_abs_diff": float(np.max(np.abs(rm.phases - rd.phases))),
             "flags_equal": bool(np.array_equal(rm.flags, rd.flags)),
             "shifts_equal": bool(np.array_equal(rm.shifts, rd.shifts)),
         }
+        del rm, rd
         log(f"mixed vs FP64 GPU: {out['mixed_vs_fp64_gpu']}")
 
     if args.other_configs:

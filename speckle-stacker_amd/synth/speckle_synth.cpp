@@ -102,11 +102,24 @@ struct Stream {
 enum Purpose : uint64_t { kScreen = 1, kNoise = 2, kObject = 3 };
 
 // --- radix-2 FFT (FP64), unnormalised, sign -1 forward --------------------
-void fft1(cd* x, int n, int stride, int sign, std::vector<cd>& w)
+// Twiddles of every stage of a length-n transform: the stage of span len holds
+// exp(sign 2 pi i m / len), m < len/2, at offset len/2 - 1.
+std::vector<cd> twiddles(int n, int sign)
 {
-    w.resize(n);
-    for (int i = 0; i < n; ++i)
-        w[i] = x[(size_t)i * stride];
+    std::vector<cd> t(std::max(n - 1, 1));
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1;
+        for (int m = 0; m < half; ++m) {
+            const double ang = 2.0 * kPi * m / len;
+            t[(size_t)(half - 1 + m)] = cd(std::cos(ang), sign * std::sin(ang));
+        }
+    }
+    return t;
+}
+
+// in-place transform of contiguous w[0, n): bit reversal, then the radix-2 stages
+void fft_core(cd* w, int n, const cd* tw)
+{
     for (int i = 1, j = 0; i < n; ++i) {
         int bit = n >> 1;
         for (; j & bit; bit >>= 1)
@@ -118,26 +131,48 @@ void fft1(cd* x, int n, int stride, int sign, std::vector<cd>& w)
     for (int len = 2; len <= n; len <<= 1) {
         const int half = len >> 1;
         for (int m = 0; m < half; ++m) {
-            const double ang = 2.0 * kPi * m / len;
-            const cd tw(std::cos(ang), sign * std::sin(ang));
+            const cd t = tw[half - 1 + m];
             for (int s = 0; s < n; s += len) {
-                const cd q = w[s + m + half] * tw;
+                const cd q = w[s + m + half] * t;
                 w[s + m + half] = w[s + m] - q;
                 w[s + m] += q;
             }
         }
     }
-    for (int i = 0; i < n; ++i)
-        x[(size_t)i * stride] = w[i];
 }
 
-void fft2(std::vector<cd>& a, int X, int Y, int sign)
+// twiddle tables of one frame shape, both directions
+struct FftTables {
+    std::vector<cd> x[2], y[2]; // [0]: sign -1, [1]: sign +1
+    FftTables(int X, int Y)
+    {
+        x[0] = twiddles(X, -1);
+        x[1] = twiddles(X, +1);
+        y[0] = twiddles(Y, -1);
+        y[1] = twiddles(Y, +1);
+    }
+};
+
+void fft2(std::vector<cd>& a, int X, int Y, int sign, const FftTables& T)
 {
-    std::vector<cd> w;
+    const int d = sign < 0 ? 0 : 1;
     for (int r = 0; r < Y; ++r)
-        fft1(a.data() + (size_t)r * X, X, 1, sign, w);
-    for (int c = 0; c < X; ++c)
-        fft1(a.data() + c, Y, X, sign, w);
+        fft_core(a.data() + (size_t)r * X, X, T.x[d].data());
+    // columns in groups of kCols, gathered into contiguous buffers: each gathered
+    // row segment is one or two cache lines instead of one line per element
+    constexpr int kCols = 8;
+    std::vector<cd> w((size_t)kCols * Y);
+    for (int c0 = 0; c0 < X; c0 += kCols) {
+        const int nc = std::min(kCols, X - c0);
+        for (int yy = 0; yy < Y; ++yy)
+            for (int c = 0; c < nc; ++c)
+                w[(size_t)c * Y + yy] = a[(size_t)yy * X + c0 + c];
+        for (int c = 0; c < nc; ++c)
+            fft_core(w.data() + (size_t)c * Y, Y, T.y[d].data());
+        for (int yy = 0; yy < Y; ++yy)
+            for (int c = 0; c < nc; ++c)
+                a[(size_t)yy * X + c0 + c] = w[(size_t)c * Y + yy];
+    }
 }
 
 struct Source {
@@ -192,35 +227,52 @@ Object make_object(const smip_synth_params& p)
     return o;
 }
 
-std::vector<double> speckle_psf(const smip_synth_params& p, uint64_t frame)
+// What every frame of one parameter set shares: the transform tables, the phase
+// screen's filter amplitude per frequency and the pupil's pixels.
+struct Setup {
+    FftTables fft;
+    std::vector<double> amp; // sqrt of the phase power spectrum; < 0 at f = 0
+    double den = 1.0;        // sqrt(L Ly), the screen's normalisation
+    std::vector<size_t> pupil;
+    explicit Setup(const smip_synth_params& p)
+        : fft(p.ncols, p.nrows)
+    {
+        const int X = p.ncols, Y = p.nrows;
+        // phase screen by FFT filtering; pupil diameter D = 1 spans D_p pixels
+        const double L = X / p.pupil_diameter, Ly = Y / p.pupil_diameter;
+        const double r0 = 1.0 / p.d_over_r0;
+        amp.resize((size_t)X * Y);
+        for (int ky = 0; ky < Y; ++ky)
+            for (int kx = 0; kx < X; ++kx) {
+                const double fx = (kx <= X / 2 ? kx : kx - X) / L, fy = (ky <= Y / 2 ? ky : ky - Y) / Ly;
+                const double f = std::hypot(fx, fy);
+                const double psd = 0.023 * std::pow(r0, -5.0 / 3.0) * std::pow(f, -11.0 / 3.0);
+                amp[(size_t)ky * X + kx] = f == 0 ? -1.0 : std::sqrt(psd);
+            }
+        den = std::sqrt(L * Ly);
+        const double cx = X / 2.0, cy = Y / 2.0, rp = p.pupil_diameter / 2.0;
+        for (int yy = 0; yy < Y; ++yy)
+            for (int xx = 0; xx < X; ++xx)
+                if (std::hypot(xx + 0.5 - cx, yy + 0.5 - cy) <= rp)
+                    pupil.push_back((size_t)yy * X + xx);
+    }
+};
+
+std::vector<double> speckle_psf(const smip_synth_params& p, const Setup& su, uint64_t frame)
 {
     const int X = p.ncols, Y = p.nrows;
     const size_t n = (size_t)X * Y;
     Stream s(p.seed, frame + 1, kScreen);
-    // phase screen by FFT filtering; pupil diameter D = 1 spans D_p pixels
-    const double L = X / p.pupil_diameter, Ly = Y / p.pupil_diameter;
-    const double r0 = 1.0 / p.d_over_r0;
     std::vector<cd> g(n);
-    for (int ky = 0; ky < Y; ++ky)
-        for (int kx = 0; kx < X; ++kx) {
-            const double fx = (kx <= X / 2 ? kx : kx - X) / L, fy = (ky <= Y / 2 ? ky : ky - Y) / Ly;
-            const double f = std::hypot(fx, fy);
-            const double re = s.gauss(), im = s.gauss();
-            if (f == 0) {
-                g[(size_t)ky * X + kx] = 0;
-                continue;
-            }
-            const double psd = 0.023 * std::pow(r0, -5.0 / 3.0) * std::pow(f, -11.0 / 3.0);
-            g[(size_t)ky * X + kx] = cd(re, im) * std::sqrt(psd) / std::sqrt(L * Ly);
-        }
-    fft2(g, X, Y, +1);
+    for (size_t i = 0; i < n; ++i) {
+        const double re = s.gauss(), im = s.gauss();
+        g[i] = su.amp[i] < 0 ? cd(0.0) : cd(re, im) * su.amp[i] / su.den;
+    }
+    fft2(g, X, Y, +1, su.fft);
     std::vector<cd> e(n, 0.0);
-    const double cx = X / 2.0, cy = Y / 2.0, rp = p.pupil_diameter / 2.0;
-    for (int yy = 0; yy < Y; ++yy)
-        for (int xx = 0; xx < X; ++xx)
-            if (std::hypot(xx + 0.5 - cx, yy + 0.5 - cy) <= rp)
-                e[(size_t)yy * X + xx] = std::polar(1.0, g[(size_t)yy * X + xx].real());
-    fft2(e, X, Y, -1);
+    for (size_t i : su.pupil)
+        e[i] = std::polar(1.0, g[i].real());
+    fft2(e, X, Y, -1, su.fft);
     std::vector<double> psf(n);
     double tot = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -233,11 +285,12 @@ std::vector<double> speckle_psf(const smip_synth_params& p, uint64_t frame)
     return psf;
 }
 
-void make_frame(const smip_synth_params& p, const Object& o, const std::vector<cd>* obj_ft, uint64_t f, void* out)
+void make_frame(const smip_synth_params& p, const Setup& su, const Object& o, const std::vector<cd>* obj_ft,
+    uint64_t f, void* out)
 {
     const int X = p.ncols, Y = p.nrows;
     const size_t n = (size_t)X * Y;
-    const std::vector<double> psf = speckle_psf(p, f);
+    const std::vector<double> psf = speckle_psf(p, su, f);
     std::vector<double> img(n, 0.0);
     if (o.image.empty()) {
         double ftot = 0;
@@ -253,10 +306,10 @@ void make_frame(const smip_synth_params& p, const Object& o, const std::vector<c
         }
     } else {
         std::vector<cd> a(psf.begin(), psf.end());
-        fft2(a, X, Y, -1);
+        fft2(a, X, Y, -1, su.fft);
         for (size_t i = 0; i < n; ++i)
             a[i] *= (*obj_ft)[i];
-        fft2(a, X, Y, +1);
+        fft2(a, X, Y, +1, su.fft);
         double tot = 0;
         for (size_t i = 0; i < n; ++i) {
             img[i] = std::max(0.0, a[i].real());
@@ -292,10 +345,11 @@ extern "C" int smip_synth_frames(const smip_synth_params* p, uint64_t first, uin
         || p->object > 2)
         return 1;
     const Object o = make_object(*p);
+    const Setup su(*p);
     std::vector<cd> obj_ft;
     if (!o.image.empty()) {
         obj_ft.assign(o.image.begin(), o.image.end());
-        fft2(obj_ft, p->ncols, p->nrows, -1);
+        fft2(obj_ft, p->ncols, p->nrows, -1, su.fft);
     }
     const size_t fb = (size_t)p->ncols * p->nrows * (p->dtype == 1 ? 1 : p->dtype == 2 ? 2 : 4);
     const int nt = std::max(1, std::min<int>(threads > 0 ? threads : 1, (int)std::max<uint64_t>(count, 1)));
@@ -303,7 +357,7 @@ extern "C" int smip_synth_frames(const smip_synth_params* p, uint64_t first, uin
     for (int t = 0; t < nt; ++t)
         pool.emplace_back([&, t] {
             for (uint64_t k = (uint64_t)t; k < count; k += (uint64_t)nt)
-                make_frame(*p, o, &obj_ft, first + k, static_cast<uint8_t*>(out) + k * fb);
+                make_frame(*p, su, o, &obj_ft, first + k, static_cast<uint8_t*>(out) + k * fb);
         });
     for (auto& th : pool)
         th.join();
