@@ -105,6 +105,8 @@ def lib(fast: bool = False) -> C.CDLL:
     L.so_spiral_visits.restype = C.c_int64
     L.so_reconstruct_phases.argtypes = [C.POINTER(Desc), P, C.c_int, C.c_uint64, C.c_uint64,
                                         C.c_double, P, P, P, C.POINTER(RecoStats)]
+    L.so_reconstruct_phases_alias.argtypes = [C.POINTER(Desc), P, C.c_int, C.c_uint64, C.c_uint64,
+                                              C.c_double, P, P, P, C.POINTER(RecoStats), C.c_int]
     L.so_xcorr.argtypes = [P, P, C.c_int64, C.c_int64, P]
     L.so_displacement.argtypes = [P, C.c_int64, C.c_int64, P]
     L.so_shifted.argtypes = [P, C.c_int64, C.c_int64, C.c_int32, C.c_int32, P]
@@ -252,13 +254,17 @@ class Reco:
     stats: dict
 
 
-def reconstruct_phases(d: Desc, bispec: np.ndarray, xsize: int, ysize: int, reco_radius: float) -> Reco:
+def reconstruct_phases(d: Desc, bispec: np.ndarray, xsize: int, ysize: int, reco_radius: float,
+                       alias_mode: int = 0) -> Reco:
+    """alias_mode 1 leaves the aliased (T3/T9) lookups out of each mean: a
+    sensitivity probe, not the reference (alias_mode 0)."""
     ph = np.zeros((ysize, xsize), dtype=np.complex128)
     fl = np.zeros((ysize, xsize), dtype=np.uint8)
     co = np.zeros((ysize, xsize), dtype=np.float64)
     st = RecoStats()
-    rc = lib().so_reconstruct_phases(C.byref(d), _p(bispec), int(bispec.dtype == np.complex64),
-                                     xsize, ysize, reco_radius, _p(ph), _p(fl), _p(co), C.byref(st))
+    rc = lib().so_reconstruct_phases_alias(C.byref(d), _p(bispec), int(bispec.dtype == np.complex64),
+                                           xsize, ysize, reco_radius, _p(ph), _p(fl), _p(co), C.byref(st),
+                                           int(alias_mode))
     _check(rc, "reconstruct_phases")
     stats = {k: int(getattr(st, k)) for k, _ in RecoStats._fields_}
     return Reco(ph, fl, co, stats)

@@ -510,9 +510,12 @@ static inline uint64_t so_pm_at(const so_pm* pm, int32_t x, int32_t y)
     return so_at(pm->xs, pm->ys, x, y);
 }
 
-/* calc_phase, include/phasereco.h:75-123 */
+/* calc_phase, include/phasereco.h:75-123.  alias_mode 0 is the reference;
+ * alias_mode 1 (a sensitivity probe, not the reference) leaves out every lookup
+ * whose address calc_offset forms from out-of-range indices, as if its |beta|
+ * were below DBL_EPSILON. */
 static int so_calc_phase(const so_desc* d, const void* B, int is_f32, so_pm* pm,
-    int32_t wx, int32_t wy, so_reco_stats* st, double* sbuf, int64_t sbuf_cap)
+    int32_t wx, int32_t wy, so_reco_stats* st, double* sbuf, int64_t sbuf_cap, int alias_mode)
 {
     /* bispec_u_range / bispec_v_range from the descriptor, :81-82 */
     if (abs(wx) + abs(wy) <= 1 || wx < d->min_idx[0] || wx > d->max_idx[0] || wy < d->min_idx[1]
@@ -542,6 +545,8 @@ static int so_calc_phase(const so_desc* d, const void* B, int is_f32, so_pm* pm,
             }
             if (rc)
                 return rc; /* get_element throws std::out_of_range */
+            if (alias && alias_mode == 1)
+                continue;
             if (is_f32) {
                 beta[0] = ((const float*)B)[2 * addr];
                 beta[1] = ((const float*)B)[2 * addr + 1];
@@ -603,6 +608,13 @@ int so_reconstruct_phases(const so_desc* d, const void* B, int is_f32,
     uint64_t xsize, uint64_t ysize, double reco_radius,
     double* phases, uint8_t* flags, double* consistency, so_reco_stats* stats)
 {
+    return so_reconstruct_phases_alias(d, B, is_f32, xsize, ysize, reco_radius, phases, flags, consistency, stats, 0);
+}
+
+int so_reconstruct_phases_alias(const so_desc* d, const void* B, int is_f32,
+    uint64_t xsize, uint64_t ysize, double reco_radius,
+    double* phases, uint8_t* flags, double* consistency, so_reco_stats* stats, int alias_mode)
+{
     so_pm pm;
     pm.xs = (int64_t)xsize;
     pm.ys = (int64_t)ysize;
@@ -635,7 +647,7 @@ int so_reconstruct_phases(const so_desc* d, const void* B, int is_f32,
         st.visits++;
         if (so_pm_contains(&pm, i, j) && !flags[so_pm_at(&pm, i, j)]) {
             st.calls++;
-            rc = so_calc_phase(d, B, is_f32, &pm, i, j, &st, sbuf, cap);
+            rc = so_calc_phase(d, B, is_f32, &pm, i, j, &st, sbuf, cap, alias_mode);
             if (rc)
                 break;
         }

@@ -92,6 +92,13 @@ typedef struct so_reco_stats {
 int so_reconstruct_phases(const so_desc* d, const void* bispec, int is_f32,
     uint64_t xsize, uint64_t ysize, double reco_radius,
     double* phases, uint8_t* flags, double* consistency, so_reco_stats* stats);
+/* The same with alias_mode 1: lookups that resolve through out-of-range indices
+ * (the T3/T9 aliasing of calc_offset's wrap) are left out of the mean -- a
+ * sensitivity probe of how much the reference's outputs depend on them, not a
+ * restatement of the reference.  alias_mode 0 is so_reconstruct_phases. */
+int so_reconstruct_phases_alias(const so_desc* d, const void* bispec, int is_f32,
+    uint64_t xsize, uint64_t ysize, double reco_radius,
+    double* phases, uint8_t* flags, double* consistency, so_reco_stats* stats, int alias_mode);
 
 /* CrossCorrelation<double>::correlate, include/crosscorrel.h:96-143 */
 int so_xcorr(const double* ref, const double* frame, int64_t ncols, int64_t nrows, double* corr);
