@@ -6,6 +6,7 @@ Produces, next to the sources:
   speckle-stacker_amd/lib/smip_cli            smip-cli compatible driver
   speckle-stacker_amd/lib/smip_dropin_test    C++ drop-in surface test runner
   speckle-stacker_amd/lib/smip_gif_dump       the CLI's GIF decoder, for the tests
+  speckle-stacker_amd/lib/smip_frame_dump     the CLI's PNG / GIF frame reading, for the tests
   speckle-stacker_amd/lib/smip_copy_calib     copy kernel calibrating rocprofv3 byte counters
 
 Run ``python speckle-stacker_amd/build.py`` or call ``build()``.
@@ -92,7 +93,12 @@ def build_cli() -> list[str]:
     dump = os.path.join(REPO, "tests", "cpp", "gif_dump.cpp")
     if os.path.exists(dump):
         out = os.path.join(LIB, "smip_gif_dump")
-        _run(["g++", dump, "-o", out, "-std=c++20", "-O2", f"-I{INCLUDE}", f"-I{PKG}"])
+        _run(["g++", dump, "-o", out, "-std=c++20", "-O2", f"-I{INCLUDE}", f"-I{PKG}", "-lz"])
+        built.append(out)
+    fdump = os.path.join(REPO, "tests", "cpp", "frame_dump.cpp")
+    if os.path.exists(fdump):
+        out = os.path.join(LIB, "smip_frame_dump")
+        _run(["g++", fdump, "-o", out, "-std=c++20", "-O2", f"-I{INCLUDE}", f"-I{PKG}", "-lz"])
         built.append(out)
     return built
 

@@ -1,5 +1,6 @@
 // frame_source.h -- frame stacks for the CLI without OpenCV: animated GIF
-// (decoded and composited here), NumPy .npy stacks and headerless raw stacks.
+// (decoded and composited here), PNG images and numbered PNG sequences (decoded
+// here with zlib), NumPy .npy stacks and headerless raw stacks.
 //
 // Channel selection follows the reference's Mat2Array (include/videoio.h:71-98):
 // a colour mask r=4, g=2, b=1, i=7 picks channel round(log2(mask)) of the
@@ -16,6 +17,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <zlib.h>
 
 #include "smip_mi355x.h"
 
@@ -381,5 +385,336 @@ inline std::unique_ptr<FrameSource> open_npy(const std::string& path)
     const uint64_t Y = dims[dims.size() - 2], X = dims.back();
     return std::make_unique<RawSource>(path, offset, X, Y, dtype);
 }
+
+// ------------------------------------------------------- PNG and PNG sequences
+// One PNG file, or a numbered sequence read as cv::VideoCapture's image-sequence
+// backend reads one (src/videoio.cpp:13-20 opens whatever VideoCapture opens): a
+// printf pattern such as "frame_%04d.png" starts at the first index in [0, 1000]
+// whose file exists; a file name whose stem ends in digits ("frame_0007.png")
+// starts at that number with the same zero padding; either continues until an
+// index has no file.  A name without digits is a single frame.  Every frame
+// must have the first one's size, colour type and bit depth.
+//
+// Decoding: all colour types (grey, grey + alpha, RGB, RGB + alpha, palette),
+// bit depths 1-16, Adam7 interlacing and the five row filters; chunk CRCs are
+// checked.  Grey samples below 8 bits are scaled to 0..255 as image readers do;
+// 16-bit images give uint16 frames (the reference's Mat2Array reads 8-bit
+// images only, include/videoio.h:75), others uint8.  Channel selection: a grey
+// image always yields its grey value; colour images follow channel_index over
+// B,G,R (alpha ignored).
+class PngImage {
+public:
+    uint32_t w = 0, h = 0;
+    int depth = 0, ctype = 0, interlace = 0;
+    std::vector<uint8_t> plte; // R,G,B triples
+
+    bool grey() const { return ctype == 0 || ctype == 4; }
+    int channels() const { return ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : 4; }
+
+    // Decodes the file; `sel` (an index in B,G,R order) picks the channel of a
+    // colour image; writes w*h samples (uint8, or uint16 when depth == 16).
+    void decode(const std::string& path, int sel, void* out)
+    {
+        const std::vector<uint8_t> f = read_file(path);
+        std::vector<uint8_t> z = parse(f, path);
+        const int spp = channels(); // samples per pixel in the file
+        const size_t bpp_bits = (size_t)spp * depth;
+        const size_t bpp = std::max<size_t>(1, bpp_bits / 8); // filter byte distance
+        // passes: (x0, y0, dx, dy); one full pass when not interlaced
+        static const int adam7[7][4] = { { 0, 0, 8, 8 }, { 4, 0, 8, 8 }, { 0, 4, 4, 8 }, { 2, 0, 4, 4 },
+            { 0, 2, 2, 4 }, { 1, 0, 2, 2 }, { 0, 1, 1, 2 } };
+        const int full[1][4] = { { 0, 0, 1, 1 } };
+        const int npass = interlace ? 7 : 1;
+        // inflated size
+        size_t need = 0;
+        for (int q = 0; q < npass; ++q) {
+            const int* P = interlace ? adam7[q] : full[0];
+            const size_t pw = w > (uint32_t)P[0] ? (w - P[0] + P[2] - 1) / P[2] : 0;
+            const size_t ph = h > (uint32_t)P[1] ? (h - P[1] + P[3] - 1) / P[3] : 0;
+            if (pw && ph)
+                need += ph * (1 + (pw * bpp_bits + 7) / 8);
+        }
+        std::vector<uint8_t> raw(need);
+        inflate_all(z, raw, path);
+        size_t pos = 0;
+        std::vector<uint8_t> prev, cur;
+        for (int q = 0; q < npass; ++q) {
+            const int* P = interlace ? adam7[q] : full[0];
+            const size_t pw = w > (uint32_t)P[0] ? (w - P[0] + P[2] - 1) / P[2] : 0;
+            const size_t ph = h > (uint32_t)P[1] ? (h - P[1] + P[3] - 1) / P[3] : 0;
+            if (!pw || !ph)
+                continue;
+            const size_t rb = (pw * bpp_bits + 7) / 8;
+            prev.assign(rb, 0);
+            cur.resize(rb);
+            for (size_t y = 0; y < ph; ++y) {
+                const uint8_t ft = raw[pos++];
+                std::memcpy(cur.data(), raw.data() + pos, rb);
+                pos += rb;
+                unfilter(ft, cur, prev, bpp, path);
+                for (size_t x = 0; x < pw; ++x)
+                    store(cur, x, spp, sel, out, (size_t)(P[1] + y * P[3]) * w + P[0] + x * P[2]);
+                std::swap(prev, cur);
+            }
+        }
+    }
+
+    // header of the file only (IHDR, PLTE)
+    void header(const std::string& path) { parse(read_file(path), path); }
+
+private:
+    static uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+    std::vector<uint8_t> parse(const std::vector<uint8_t>& f, const std::string& path)
+    {
+        static const uint8_t sig[8] = { 0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n' };
+        if (f.size() < 8 || std::memcmp(f.data(), sig, 8))
+            throw std::runtime_error("PNG: bad signature: " + path);
+        std::vector<uint8_t> z;
+        bool ihdr = false, iend = false;
+        for (size_t p = 8; p + 12 <= f.size() && !iend;) {
+            const uint32_t len = be32(&f[p]);
+            if (p + 12 + (size_t)len > f.size())
+                throw std::runtime_error("PNG: truncated chunk: " + path);
+            const uint8_t* type = &f[p + 4];
+            const uint8_t* body = &f[p + 8];
+            const uint32_t crc = (uint32_t)crc32(crc32(0L, Z_NULL, 0), type, 4 + len);
+            if (crc != be32(&f[p + 8 + len]))
+                throw std::runtime_error("PNG: chunk CRC mismatch: " + path);
+            if (!std::memcmp(type, "IHDR", 4)) {
+                if (len != 13)
+                    throw std::runtime_error("PNG: bad IHDR: " + path);
+                w = be32(body);
+                h = be32(body + 4);
+                depth = body[8];
+                ctype = body[9];
+                interlace = body[12];
+                const bool ok = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16))
+                    || (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8))
+                    || ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+                if (!ok || body[10] != 0 || body[11] != 0 || interlace > 1 || w == 0 || h == 0)
+                    throw std::runtime_error("PNG: unsupported IHDR: " + path);
+                ihdr = true;
+            } else if (!std::memcmp(type, "PLTE", 4)) {
+                plte.assign(body, body + len);
+            } else if (!std::memcmp(type, "IDAT", 4)) {
+                z.insert(z.end(), body, body + len);
+            } else if (!std::memcmp(type, "IEND", 4)) {
+                iend = true;
+            }
+            p += 12 + (size_t)len;
+        }
+        if (!ihdr || z.empty())
+            throw std::runtime_error("PNG: missing IHDR or IDAT: " + path);
+        if (ctype == 3 && (plte.empty() || plte.size() % 3))
+            throw std::runtime_error("PNG: palette image without a valid PLTE: " + path);
+        return z;
+    }
+
+    static void inflate_all(std::vector<uint8_t>& z, std::vector<uint8_t>& raw, const std::string& path)
+    {
+        z_stream st {};
+        if (inflateInit(&st) != Z_OK)
+            throw std::runtime_error("PNG: zlib init failed");
+        st.next_in = z.data();
+        st.avail_in = (uInt)z.size();
+        st.next_out = raw.data();
+        st.avail_out = (uInt)raw.size();
+        const int rc = inflate(&st, Z_FINISH);
+        const size_t got = raw.size() - st.avail_out;
+        inflateEnd(&st);
+        if ((rc != Z_STREAM_END && rc != Z_BUF_ERROR) || got != raw.size())
+            throw std::runtime_error("PNG: image data shorter than its header says: " + path);
+    }
+
+    static void unfilter(uint8_t ft, std::vector<uint8_t>& cur, const std::vector<uint8_t>& prev, size_t bpp,
+        const std::string& path)
+    {
+        const size_t n = cur.size();
+        switch (ft) {
+        case 0:
+            break;
+        case 1: // Sub
+            for (size_t i = bpp; i < n; ++i)
+                cur[i] = (uint8_t)(cur[i] + cur[i - bpp]);
+            break;
+        case 2: // Up
+            for (size_t i = 0; i < n; ++i)
+                cur[i] = (uint8_t)(cur[i] + prev[i]);
+            break;
+        case 3: // Average
+            for (size_t i = 0; i < n; ++i)
+                cur[i] = (uint8_t)(cur[i] + (((i >= bpp ? cur[i - bpp] : 0) + prev[i]) >> 1));
+            break;
+        case 4: // Paeth
+            for (size_t i = 0; i < n; ++i) {
+                const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+                const int pp = a + b - c, pa = std::abs(pp - a), pb = std::abs(pp - b), pc = std::abs(pp - c);
+                cur[i] = (uint8_t)(cur[i] + (pa <= pb && pa <= pc ? a : pb <= pc ? b : c));
+            }
+            break;
+        default:
+            throw std::runtime_error("PNG: unknown row filter: " + path);
+        }
+    }
+
+    // sample k (0-based within the row's samples) at `depth` bits
+    uint32_t sample(const std::vector<uint8_t>& row, size_t k) const
+    {
+        if (depth == 8)
+            return row[k];
+        if (depth == 16)
+            return (uint32_t)row[2 * k] << 8 | row[2 * k + 1];
+        const size_t bit = k * depth;
+        return (row[bit >> 3] >> (8 - depth - (bit & 7))) & ((1u << depth) - 1);
+    }
+
+    void store(const std::vector<uint8_t>& row, size_t x, int spp, int sel, void* out, size_t at) const
+    {
+        uint32_t v;
+        if (ctype == 3) { // palette index -> R,G,B; sel counts B,G,R
+            const size_t idx = sample(row, x);
+            if (3 * idx + 2 >= plte.size())
+                throw std::runtime_error("PNG: palette index beyond PLTE");
+            v = plte[3 * idx + (2 - sel)];
+        } else if (grey()) {
+            v = sample(row, x * spp);
+            if (depth < 8)
+                v = v * 255u / ((1u << depth) - 1u);
+        } else {
+            v = sample(row, x * spp + (2 - sel));
+        }
+        if (depth == 16)
+            static_cast<uint16_t*>(out)[at] = (uint16_t)v;
+        else
+            static_cast<uint8_t*>(out)[at] = (uint8_t)v;
+    }
+};
+
+class PngSource : public FrameSource {
+public:
+    PngSource(const std::string& path, int colour_mask)
+    {
+        m_files = sequence(path);
+        if (m_files.empty())
+            throw std::runtime_error("file open error: " + path);
+        m_first.header(m_files[0]);
+        m_sel = channel_index(colour_mask, 3);
+    }
+    uint64_t nframes() const override { return m_files.size(); }
+    uint64_t ncols() const override { return m_first.w; }
+    uint64_t nrows() const override { return m_first.h; }
+    int32_t dtype() const override { return m_first.depth == 16 ? SMIP_DTYPE_U16 : SMIP_DTYPE_U8; }
+    void read(uint64_t first, uint64_t count, void* out) override
+    {
+        for (uint64_t f = 0; f < count; ++f) {
+            PngImage im;
+            const std::string& path = m_files.at(first + f);
+            im.header(path);
+            if (im.w != m_first.w || im.h != m_first.h || im.depth != m_first.depth || im.ctype != m_first.ctype)
+                throw std::runtime_error("PNG sequence: " + path + " differs in size or pixel format from "
+                    + m_files[0]);
+            im.decode(path, m_sel, static_cast<uint8_t*>(out) + f * frame_bytes());
+        }
+    }
+    const std::vector<std::string>& files() const { return m_files; }
+
+    // The files of a path or pattern, in order.
+    static std::vector<std::string> sequence(const std::string& path)
+    {
+        auto exists = [](const std::string& p) {
+            struct stat sb;
+            return ::stat(p.c_str(), &sb) == 0 && S_ISREG(sb.st_mode);
+        };
+        std::vector<std::string> out;
+        Pattern pat;
+        long start = 0;
+        if (path.find('%') != std::string::npos) {
+            pat = parse_pattern(path);
+            start = -1;
+            for (long i = 0; i <= 1000 && start < 0; ++i)
+                if (exists(pat.render(i)))
+                    start = i;
+            if (start < 0)
+                return out;
+        } else {
+            // "<prefix><digits>.<ext>": a sequence from that number, same width
+            const size_t slash = path.find_last_of('/');
+            const size_t base = slash == std::string::npos ? 0 : slash + 1;
+            const size_t dot = path.rfind('.');
+            const size_t stem_end = (dot == std::string::npos || dot < base) ? path.size() : dot;
+            size_t d0 = stem_end;
+            while (d0 > base && std::isdigit(static_cast<unsigned char>(path[d0 - 1])))
+                --d0;
+            if (!exists(path))
+                return out;
+            if (d0 == stem_end || stem_end - d0 > 9) {
+                out.push_back(path);
+                return out;
+            }
+            pat.pre = path.substr(0, d0);
+            pat.post = path.substr(stem_end);
+            pat.width = (int)(stem_end - d0);
+            pat.zero = true;
+            start = std::stol(path.substr(d0, stem_end - d0));
+        }
+        for (long i = start; exists(pat.render(i)); ++i)
+            out.push_back(pat.render(i));
+        return out;
+    }
+
+private:
+    // "<pre>%[0][width][l]d<post>", with "%%" for a literal '%' elsewhere
+    struct Pattern {
+        std::string pre, post;
+        int width = 0;
+        bool zero = false;
+        std::string render(long i) const
+        {
+            std::string d = std::to_string(i);
+            if ((int)d.size() < width)
+                d.insert(0, (size_t)(width - (int)d.size()), zero ? '0' : ' ');
+            return pre + d + post;
+        }
+    };
+    static Pattern parse_pattern(const std::string& p)
+    {
+        Pattern r;
+        bool seen = false;
+        std::string* cur = &r.pre;
+        for (size_t k = 0; k < p.size(); ++k) {
+            if (p[k] != '%') {
+                *cur += p[k];
+                continue;
+            }
+            if (k + 1 < p.size() && p[k + 1] == '%') {
+                *cur += '%';
+                ++k;
+                continue;
+            }
+            size_t e = k + 1;
+            if (e < p.size() && p[e] == '0') {
+                r.zero = true;
+                ++e;
+            }
+            while (e < p.size() && std::isdigit(static_cast<unsigned char>(p[e])))
+                r.width = r.width * 10 + (p[e++] - '0');
+            while (e < p.size() && p[e] == 'l')
+                ++e;
+            if (seen || e >= p.size() || (p[e] != 'd' && p[e] != 'u') || r.width > 32)
+                throw std::runtime_error("image sequence pattern needs one %d conversion: " + p);
+            seen = true;
+            cur = &r.post;
+            k = e;
+        }
+        if (!seen)
+            throw std::runtime_error("image sequence pattern needs one %d conversion: " + p);
+        return r;
+    }
+
+    std::vector<std::string> m_files;
+    PngImage m_first;
+    int m_sel = 2;
+};
 
 } // namespace smip_cli

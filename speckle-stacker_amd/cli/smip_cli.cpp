@@ -130,6 +130,8 @@ std::unique_ptr<FrameSource> open_source(const std::string& path, const std::str
     std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
     if (ext == "npy")
         return open_npy(path);
+    if (ext == "png")
+        return std::make_unique<PngSource>(path, colour);
     return std::make_unique<GifSource>(path, colour);
 }
 
