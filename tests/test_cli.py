@@ -269,7 +269,14 @@ def test_cli_reference_example(tmp_path, precision):
     ref = np.asarray(Image.open(os.path.join(GOLDEN, "hu940ani_reco_image.png"))).astype(np.float64)
     ours = read_png16(os.path.join(tmp_path, "reco_image.png")).astype(np.float64)
     assert np.unravel_index(ours.argmax(), ours.shape) == np.unravel_index(ref.argmax(), ref.shape) == (64, 64)
-    assert np.corrcoef(ours.ravel(), ref.ravel())[0, 1] > 0.98
+    # The oracle on PIL-decoded frames is 143 LSB (median), 562 (99th percentile)
+    # and 1245 (max) from this file, correlation 0.991; the engine's image is
+    # within 1 LSB of the oracle's (above), so it carries the same distance.
+    # Leaving the aliased lookups out would move it to 203 / 813 / 3996.
+    d = np.abs(ours - ref)
+    print(f"reference reco_image.png: median {np.median(d):.0f} LSB, p99 {np.percentile(d, 99):.0f}, max {d.max():.0f}")
+    assert np.median(d) <= 160 and np.percentile(d, 99) <= 650 and d.max() <= 1400
+    assert np.corrcoef(ours.ravel(), ref.ravel())[0, 1] > 0.99
 
 
 @pytest.mark.gpu
