@@ -378,7 +378,10 @@ def draw_sequences(rng, n: int) -> list:
     return out
 
 
-def frame_pools(ctx, host: bool):
+def frame_pools(ctx, host: bool, length: int = 128):
+    """64 distinct frames per (side, type); on the device repeated to `length`
+    frames, so that frames off .. off + n - 1 (off < 64) of a pool are the
+    sequence's frames off, off + 1, ... cycling through the 64."""
     pools = {}
     for N in (64, 128, 256, 512):
         for kind in ("binary128", "triple256", "extended512"):
@@ -386,7 +389,13 @@ def frame_pools(ctx, host: bool):
             prm.ncols = prm.nrows = N
             prm.pupil_diameter = max(8.0, N / 4)
             f = S.synth_frames(prm, 0, 64, threads=min(16, os.cpu_count() or 1))
-            pools[(N, kind)] = f if host else S.Frames(ctx, np.concatenate([f, f]))  # 128: any 64-frame window
+            if host:
+                pools[(N, kind)] = f
+                continue
+            dev = S.Frames(ctx, dtype=f.dtype, shape=(length, N, N))
+            for at in range(0, length, 64):
+                dev.upload(at, f[:min(64, length - at)])
+            pools[(N, kind)] = dev
     return pools
 
 
@@ -396,7 +405,7 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
     device-resident pools (64 distinct frames per side and type, cycled)."""
     rng = np.random.default_rng(0x5EC1E004)
     seqs = draw_sequences(rng, nseq)
-    pools = frame_pools(ctx, host=False)
+    pools = frame_pools(ctx, host=False, length=max(q["n"] for q in seqs) + 64)
     lat = [0.0] * nseq
     errs = []
     nxt = [0]
@@ -406,11 +415,7 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
         dev = pools[(q["N"], q["kind"])]
         job = S.Job(ctx, S.job_params(q["N"], q["N"], q["b"], q["p"], dev.dtype, precision))
         n = q["n"] if limit is None else min(limit, q["n"])
-        pos = 0
-        while pos < n:
-            c = min(64, n - pos)
-            job.add_frames(dev, (q["off"] + pos) % 64, c)
-            pos += c
+        job.add_frames(dev, q["off"], n)
         job.finalize()
         st = job.stats()
         job.close()
@@ -436,11 +441,7 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
                 q = seqs[i]
                 dev = pools[(q["N"], q["kind"])]
                 job = S.Job(ctx, S.job_params(q["N"], q["N"], q["b"], q["p"], dev.dtype, S.PRECISION_MIXED))
-                pos = 0
-                while pos < q["n"]:
-                    c = min(64, q["n"] - pos)
-                    job.add_frames(dev, (q["off"] + pos) % 64, c)
-                    pos += c
+                job.add_frames(dev, q["off"], q["n"])  # batch by batch, FFT of the next beside the current
                 job.synchronize()
                 with cv:
                     ready.append((i, job))
@@ -505,7 +506,141 @@ def batch_leg(ctx, nseq: int, threads: int, plain_sample: int) -> dict:
                               "to its frame count, plus its phase recovery and synthesis",
             "mix": "side {64,128,256,512}, frames log-uniform 40-5000, b U[16,64], p=2b, "
                    "binary/u8, triple/u16, extended/f32; frames cycle through 64 per side and type",
-            "order": "longest first (frames x N^2 b^2)"}
+            "order": "longest first (frames x N^2 b^2)",
+            "form": "16 host threads, each driving one sequence at a time through smip_job_add_frames on the "
+                    "sequence's own streams; grouped phase recovery"}
+
+
+def batch_leg_grouped(ctx, nseq: int, batches_per_call: int = 1, window: int = 0) -> dict:
+    """The multi-sequence mix of batch_leg through smip_job_add_frames_many: one
+    host thread advances every unfinished sequence by its next batch in one call,
+    so each stage is one launch for all of them; sequences whose frames are all
+    in go to one finalizer thread, which recovers the phases of every sequence
+    then ready in one group."""
+    rng = np.random.default_rng(0x5EC1E004)
+    seqs = draw_sequences(rng, nseq)  # the same sequences as batch_leg
+    pools = frame_pools(ctx, host=False, length=max(q["n"] for q in seqs) + 64)
+    lat = [0.0] * nseq
+    errs = []
+    ready = []
+    cv = threading.Condition()
+    groups = []
+    calls = [0]
+    window = window or nseq  # sequences advanced together at most
+
+    def finalizer():
+        done = 0
+        while done < nseq:
+            with cv:
+                cv.wait_for(lambda: ready)
+                batch = ready[:]
+                ready.clear()
+            jobs = [j for _, j in batch if j is not None]
+            try:
+                if jobs:
+                    S.finalize_many(jobs)
+                    groups.append(len(jobs))
+            except Exception as e:  # reported, not hidden
+                errs.append(repr(e))
+            t = (time.perf_counter() - t0) * 1e3
+            for i, j in batch:
+                lat[i] = t
+                if j is not None:
+                    j.close()
+            done += len(batch)
+
+    warm = [S.Job(ctx, S.job_params(q["N"], q["N"], q["b"], q["p"], pools[(q["N"], q["kind"])].dtype,
+                                    S.PRECISION_MIXED)) for q in seqs[:4]]
+    S.add_frames_many(warm, [pools[(q["N"], q["kind"])] for q in seqs[:4]], [0] * 4, [8] * 4)
+    for j in warm:
+        j.close()
+    pos = [0] * nseq
+    batch = [S.resolve_params(S.job_params(q["N"], q["N"], q["b"], q["p"], pools[(q["N"], q["kind"])].dtype,
+                                           S.PRECISION_MIXED)).frames_per_batch for q in seqs]
+    # jobs are created on a thread of their own (their device allocations are
+    # synchronous) while the driver already advances the ones made, longest first
+    # as in batch_leg
+    order = sorted(range(nseq), key=lambda k: -seqs[k]["n"] * (seqs[k]["N"] * seqs[k]["b"]) ** 2)
+    jobs = [None] * nseq
+    made = []
+    made_cv = threading.Condition()
+
+    def creator():
+        for i in order:
+            q = seqs[i]
+            try:
+                j = S.Job(ctx, S.job_params(q["N"], q["N"], q["b"], q["p"], pools[(q["N"], q["kind"])].dtype,
+                                            S.PRECISION_MIXED))
+            except Exception as e:
+                errs.append(repr(e))
+                j = None
+            with made_cv:
+                jobs[i] = j
+                made.append(i)
+                made_cv.notify()
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=finalizer)
+    th.start()
+    mk = threading.Thread(target=creator)
+    mk.start()
+    taken = 0
+    active = []
+    sent = set()
+    try:
+        while True:
+            with made_cv:
+                if not active and taken < nseq:
+                    made_cv.wait_for(lambda: len(made) > taken)
+                new = made[taken:taken + max(0, window - len(active))]
+                taken += len(new)
+            failed = [i for i in new if jobs[i] is None]
+            if failed:
+                sent.update(failed)
+                with cv:
+                    ready.extend((i, None) for i in failed)
+                    cv.notify()
+            active += [i for i in new if jobs[i] is not None]
+            if not active:
+                if taken >= nseq:
+                    break
+                continue
+            firsts = [seqs[i]["off"] + pos[i] for i in active]
+            counts = [min(batches_per_call * batch[i], seqs[i]["n"] - pos[i]) for i in active]
+            S.add_frames_many([jobs[i] for i in active], [pools[(seqs[i]["N"], seqs[i]["kind"])] for i in active],
+                              firsts, counts)
+            calls[0] += 1
+            finished = []
+            for i, c in zip(active, counts):
+                pos[i] += c
+                if pos[i] >= seqs[i]["n"]:
+                    finished.append(i)
+            if finished:
+                sent.update(finished)
+                with cv:
+                    ready.extend((i, jobs[i]) for i in finished)
+                    cv.notify()
+            active = [i for i in active if pos[i] < seqs[i]["n"]]
+    except Exception as e:
+        errs.append(repr(e))
+        mk.join()
+        with cv:  # every sequence not yet handed over ends now (its job closed unfinished)
+            ready.extend((i, jobs[i]) for i in range(nseq) if i not in sent)
+            cv.notify()
+    mk.join()
+    th.join()
+    wall = time.perf_counter() - t0
+    if errs:
+        raise RuntimeError("; ".join(errs))
+    total = sum(q["n"] for q in seqs)
+    lat = np.array(lat)
+    return {"sequences": nseq, "frames": total, "makespan_s": round(wall, 3), "grouped_calls": calls[0],
+            "phase_recovery_groups": groups, "frames_per_s": round(total / wall, 1),
+            "sequence_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
+                                    "p99": round(float(np.percentile(lat, 99)), 1)},
+            "form": f"smip_job_add_frames_many: one launch per stage and round for the next {batches_per_call} "
+                    f"batch(es) of up to {window} unfinished sequences per call (admitted longest first), one host "
+                    "thread; grouped phase recovery"}
 
 
 def phase_bins(m, d, u, fl) -> dict:
@@ -829,6 +964,15 @@ def main() -> int:
         log(f"multi-sequence: {args.batch_seqs} sequences, 16 host threads")
         out["multi_sequence"] = batch_leg(ctx, args.batch_seqs, 16, plain_sample=32)
         log(f"multi-sequence: {out['multi_sequence']}")
+        log(f"multi-sequence, grouped launches: {args.batch_seqs} sequences")
+        for bpc in (2, 4):
+            gg = batch_leg_grouped(ctx, args.batch_seqs, bpc)
+            log(f"multi-sequence, grouped launches, {bpc} batches per call: {gg}")
+        g = batch_leg_grouped(ctx, args.batch_seqs)
+        g["speedup_vs_plain_fp64_serial"] = round(out["multi_sequence"]["plain_fp64_serial_s_estimate"]
+                                                  / g["makespan_s"], 2)
+        out["multi_sequence_grouped"] = g
+        log(f"multi-sequence, grouped launches: {g}")
 
     if args.precision_frames > 0:
         log(f"precision: mixed vs FP64 on {args.precision_frames} frames of 512x512, b=64, p=256")

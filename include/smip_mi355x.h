@@ -227,6 +227,19 @@ SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p, smip_job**
 SMIP_API int smip_job_destroy(smip_job* j);
 /* Enqueue frames [first, first+count) of a device stack (asynchronous). */
 SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, uint64_t first, uint64_t count);
+/* smip_job_add_frames for n jobs at once (no counterpart in the reference, which
+ * processes one stack per process): frames [first[k], first[k] + count[k]) of
+ * stacks[k] into jobs[k].  The jobs advance in lockstep rounds, each taking the
+ * next batch of every job that has frames left -- the batches
+ * smip_job_add_frames forms -- and each round runs every stage once for all of
+ * them: the forward transform passes, |F|^2, the triple products (one launch per
+ * vy tile width) and the shift-and-add passes, each block of a launch working for
+ * one job.  Each job's results equal smip_job_add_frames's, bit for bit.  Jobs
+ * in FP64 precision or with a frame side that is not a power of two >= 8 take
+ * their own launches instead.  Jobs must share a device and appear once; the
+ * call returns when the grouped work is done. */
+SMIP_API int smip_job_add_frames_many(smip_job* const* jobs, const smip_frames* const* stacks, const uint64_t* first,
+    const uint64_t* count, uint64_t n);
 /* Upload host frames and enqueue them. */
 SMIP_API int smip_job_add_frames_host(smip_job* j, const void* frames, uint64_t count);
 /* Streaming form of the frame loop (smip-cli/main.cpp:281-306): appends count

@@ -7,6 +7,7 @@
 // exceptions to status codes.  All arithmetic on frames, spectra and the
 // bispectrum runs in the HIP kernels of fft.hip, bispec.hip, reco.hip, post.hip.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1562,8 +1563,10 @@ void enable_overlap(smip_job* j)
 {
     if (j->fstream)
         return;
-    j->H32b = dmake<float2>((size_t)j->batch * j->hp.fs);
-    SMIP_HIP(hipMemsetAsync(j->H32b.get(), 0, sizeof(float2) * j->batch * j->hp.fs, j->stream)); // padding rows
+    if (!j->H32b) { // made with the job for mixed precision; padding rows zeroed there
+        j->H32b = dmake<float2>((size_t)j->batch * j->hp.fs);
+        SMIP_HIP(hipMemsetAsync(j->H32b.get(), 0, sizeof(float2) * j->batch * j->hp.fs, j->stream));
+    }
     for (int k = 0; k < 2; ++k) {
         SMIP_HIP(hipEventCreateWithFlags(&j->ev_fft[k], hipEventDisableTiming));
         SMIP_HIP(hipEventCreateWithFlags(&j->ev_tile[k], hipEventDisableTiming));
@@ -1803,6 +1806,10 @@ extern "C" SMIP_API int smip_job_create(smip_ctx* ctx, const smip_job_params* p,
             j->tile = tile_plan_create(z.d, z.L, z.hp, z.vy_tile, s);
             j->H32 = dmake<float2>((size_t)j->batch * z.hp.fs);
             SMIP_HIP(hipMemsetAsync(j->H32.get(), 0, sizeof(float2) * j->batch * z.hp.fs, s)); // zero padding rows
+            // the second half of the double-buffered spectrum (the footprint counts
+            // both): made now rather than when the first overlapped batch needs it
+            j->H32b = dmake<float2>((size_t)j->batch * z.hp.fs);
+            SMIP_HIP(hipMemsetAsync(j->H32b.get(), 0, sizeof(float2) * j->batch * z.hp.fs, s));
             j->work32 = dmake<float2>((size_t)j->batch * hx * Y);
         } else {
             j->H64 = dmake<double2>((size_t)j->batch * z.hp.fs);
@@ -1868,6 +1875,478 @@ extern "C" SMIP_API int smip_job_add_frames(smip_job* j, const smip_frames* f, u
         grow_shifts(j, j->nframes + count);
         const size_t fb = f->X * f->Y * dtype_size(f->dtype);
         job_process(j, static_cast<const uint8_t*>(f->data) + first * fb, count);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Many jobs at once: one launch per stage for the current batch of every job
+// ---------------------------------------------------------------------------
+namespace {
+
+// Host image of one round's grouped tables, uploaded in one copy.
+struct TableImage {
+    std::vector<uint8_t> bytes;
+    template <typename T>
+    size_t add(const T* p, size_t n)
+    {
+        const size_t at = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(at + sizeof(T) * n);
+        if (n)
+            std::memcpy(bytes.data() + at, p, sizeof(T) * n);
+        return at;
+    }
+};
+
+// Two pinned + device slots for the round tables: the copy of round r + 2 into a
+// slot waits (host side) for round r's copy out of it; kernels reading a slot run
+// before the next copy into it on the same stream.
+struct TableRing {
+    uint8_t* host[2] = { nullptr, nullptr };
+    uint8_t* dev[2] = { nullptr, nullptr };
+    hipEvent_t copied[2] = { nullptr, nullptr };
+    size_t cap = 0;
+    int cur = 0;
+    explicit TableRing(size_t bytes)
+        : cap(std::max<size_t>(bytes, 4096))
+    {
+        for (int k = 0; k < 2; ++k) {
+            host[k] = static_cast<uint8_t*>(host_malloc(cap));
+            dev[k] = static_cast<uint8_t*>(dev_malloc(cap));
+            SMIP_HIP(hipEventCreateWithFlags(&copied[k], hipEventDisableTiming));
+        }
+    }
+    ~TableRing()
+    {
+        for (int k = 0; k < 2; ++k) {
+            if (copied[k])
+                (void)hipEventDestroy(copied[k]);
+            host_free(host[k]);
+            dev_free(dev[k]);
+        }
+    }
+    uint8_t* upload(const TableImage& t, hipStream_t s)
+    {
+        require(t.bytes.size() <= cap, SMIP_ERR_LOGIC, "grouped tables exceed their buffer");
+        const int k = cur;
+        cur ^= 1;
+        SMIP_HIP(hipEventSynchronize(copied[k]));
+        std::memcpy(host[k], t.bytes.data(), t.bytes.size());
+        SMIP_HIP(hipMemcpyAsync(dev[k], host[k], t.bytes.size(), hipMemcpyHostToDevice, s));
+        SMIP_HIP(hipEventRecord(copied[k], s));
+        return dev[k];
+    }
+};
+
+// Per device: the table ring and the three streams of smip_job_add_frames_many,
+// and the lock that gives them to one call at a time.  The streams live as long
+// as the ring, whose copy events they record (an event is not waited on after
+// the stream it was recorded on is gone).  Never destroyed (no HIP calls at
+// process exit).
+struct RingSlot {
+    std::mutex use;
+    std::unique_ptr<TableRing> ring;
+    hipStream_t gs = nullptr, fs = nullptr, ss = nullptr;
+};
+RingSlot& ring_slot(int dev)
+{
+    static std::mutex mu;
+    static auto* slots = new std::map<int, std::unique_ptr<RingSlot>>;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& p = (*slots)[dev];
+    if (!p)
+        p = std::make_unique<RingSlot>();
+    return *p;
+}
+std::unique_lock<std::mutex> table_ring_lock(int dev) { return std::unique_lock<std::mutex>(ring_slot(dev).use); }
+// the grouped streams of the device (caller holds its lock), made on first use
+void group_streams(int dev, hipStream_t& gs, hipStream_t& fs, hipStream_t& ss)
+{
+    RingSlot& r = ring_slot(dev);
+    for (hipStream_t* st : { &r.gs, &r.fs, &r.ss })
+        if (!*st)
+            *st = make_stream();
+    gs = r.gs;
+    fs = r.fs;
+    ss = r.ss;
+}
+// the ring of the device (caller holds its lock), at least `cap` bytes per slot;
+// a smaller ring is replaced once idle (its slots' copies have completed)
+TableRing& table_ring(int dev, size_t cap)
+{
+    RingSlot& r = ring_slot(dev);
+    if (!r.ring || r.ring->cap < cap) {
+        if (r.ring) {
+            for (hipEvent_t e : r.ring->copied)
+                SMIP_HIP(hipEventSynchronize(e));
+            IdleRelease idle; // its last copies have landed and its readers were synchronised
+            r.ring.reset();
+        }
+        r.ring = std::make_unique<TableRing>(cap + cap / 2);
+    }
+    return *r.ring;
+}
+
+struct GroupMember {
+    smip_job* j;
+    const uint8_t* base; // frames of the job's stack, from its first frame
+    uint64_t pos = 0, total = 0;
+    FftTiling t {};
+    size_t fb = 0;
+};
+
+// Offsets of one grouped launch: off[k] blocks before member k.
+struct Offsets {
+    std::vector<uint32_t> off { 0 };
+    void add(uint64_t blocks)
+    {
+        const uint64_t next = (uint64_t)off.back() + blocks;
+        require(next < (1ull << 31), SMIP_ERR_INVALID_ARGUMENT, "grouped launch too large");
+        off.push_back((uint32_t)next);
+    }
+    uint32_t total() const { return off.back(); }
+};
+
+// The lockstep rounds of smip_job_add_frames_many for jobs the grouped kernels
+// take (mixed precision, power-of-two frame sides).  Each round takes the next
+// batch of every job that has frames left -- the batches smip_job_add_frames
+// would form -- and runs each stage once for all of them: the forward transforms
+// and |F|^2 on fs into one half of each job's double-buffered spectrum, the
+// triple products on gs once those are done, and the shift-and-add on ss beside
+// both, in chunks of saa_chunk frames as the single-job path runs it.  So the
+// next round's transforms and this round's shift-and-add overlap this round's
+// triple products, as they do across the streams of separately driven jobs.
+// Synchronous.
+void run_group(std::vector<GroupMember>& M)
+{
+    const size_t n = M.size();
+    // table bytes of the largest round
+    int max_chunks = 1;
+    for (auto& m : M) {
+        const smip_job_params& p = m.j->p;
+        if (p.saa)
+            max_chunks = std::max(max_chunks, (m.j->batch + saa_chunk(p.ncols, p.nrows, m.j->batch) - 1)
+                    / saa_chunk(p.ncols, p.nrows, m.j->batch));
+        enable_overlap(m.j); // the second spectrum buffer (padding rows zeroed on the job stream)
+        SMIP_HIP(hipStreamSynchronize(m.j->stream));
+    }
+    // tables and offsets of the largest round; every class (shape, tile width,
+    // chunk index and shape) adds at most one entry per member plus one offset
+    // array terminator and alignment
+    const size_t cap = n * (sizeof(GroupFwd) + sizeof(GroupTile)) + (size_t)max_chunks * n * sizeof(GroupSaa)
+        + sizeof(uint32_t) * (4 * n + 5 * (size_t)max_chunks * n) + (n + 8) * (size_t)(max_chunks + 2) * 5 * 32;
+    // the device's table ring, kept between calls (it grows to the largest round
+    // seen); calls on one device take it in turn
+    int dev = 0;
+    SMIP_HIP(hipGetDevice(&dev));
+    std::unique_lock<std::mutex> ring_lock = table_ring_lock(dev);
+    hipStream_t gs = nullptr, fs = nullptr, ss = nullptr;
+    group_streams(dev, gs, fs, ss);
+    TableRing& ring = table_ring(dev, cap);
+    struct RoundEvents {
+        hipEvent_t f0, t0, t1, s0, s1;
+        std::vector<size_t> members;
+    };
+    std::vector<RoundEvents> rounds;
+    std::vector<hipEvent_t> sync_ev; // order-only events
+    auto order_event = [&] {
+        hipEvent_t e;
+        SMIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        sync_ev.push_back(e);
+        return e;
+    };
+    auto cleanup = [&] {
+        for (hipStream_t st : { fs, ss, gs })
+            (void)hipStreamSynchronize(st);
+        for (auto& r : rounds)
+            for (hipEvent_t e : { r.f0, r.t0, r.t1, r.s0, r.s1 })
+                (void)hipEventDestroy(e);
+        for (hipEvent_t e : sync_ev)
+            (void)hipEventDestroy(e);
+    };
+    try {
+        hipEvent_t e_up = order_event(), e_fft = order_event();
+        hipEvent_t e_tile[2] = { order_event(), order_event() };     // last reader of spectrum half h
+        hipEvent_t e_fs_done[2] = { order_event(), order_event() };  // fs's last reads of table slot k
+        hipEvent_t e_ss_done[2] = { order_event(), order_event() };  // ss's last reads of table slot k
+        bool tile_used[2] = { false, false }, slot_used[2] = { false, false };
+        auto timing_event = [&] {
+            hipEvent_t e;
+            SMIP_HIP(hipEventCreate(&e));
+            return e;
+        };
+        for (int round = 0;; ++round) {
+            std::vector<size_t> R;
+            for (size_t k = 0; k < n; ++k)
+                if (M[k].pos < M[k].total)
+                    R.push_back(k);
+            if (R.empty())
+                break;
+            const int half = round & 1;
+            std::vector<int64_t> nf(n, 0);
+            // forward transforms and |F|^2 per frame shape (a launch's dynamic LDS is
+            // its largest member's), triple products per tile width
+            struct FwdClass {
+                std::vector<GroupFwd> tab;
+                Offsets rows, cols, pow;
+                size_t lds_rows = 0, lds_cols = 0;
+            };
+            std::map<std::pair<int, int>, FwdClass> fwdc;
+            std::map<int, std::pair<std::vector<GroupTile>, Offsets>> tiles;
+            for (size_t k : R) {
+                GroupMember& m = M[k];
+                smip_job* j = m.j;
+                const smip_job_params& p = j->p;
+                nf[k] = (int64_t)std::min<uint64_t>((uint64_t)j->batch, m.total - m.pos);
+                float2* H = half ? j->H32b.get() : j->H32.get();
+                GroupFwd g {};
+                g.frames = m.base + m.pos * m.fb;
+                g.dtype = p.frame_dtype;
+                g.X = (int32_t)p.ncols;
+                g.Y = (int32_t)p.nrows;
+                g.lx = m.t.lx;
+                g.ly = m.t.ly;
+                g.hx = g.X / 2 + 1;
+                g.ppw = m.t.ppw32;
+                g.cb = m.t.cb32;
+                g.nf = nf[k];
+                g.twx = m.t.twx32;
+                g.twy = m.t.twy32;
+                g.G = j->work32.get();
+                g.H = H;
+                g.hp = j->hp;
+                g.P = j->P_half.get();
+                FwdClass& fc = fwdc[{ g.X, g.Y }];
+                fc.tab.push_back(g);
+                fc.rows.add((uint64_t)((g.Y / 2) / g.ppw) * g.nf);
+                fc.cols.add((uint64_t)((g.hx + g.cb - 1) / g.cb) * g.nf);
+                fc.pow.add(((uint64_t)g.hx * g.Y + 255) / 256);
+                fc.lds_rows = std::max(fc.lds_rows, m.t.lds_rows32);
+                fc.lds_cols = std::max(fc.lds_cols, m.t.lds_cols32);
+                GroupTile t {};
+                t.items = tile_plan_items(j->tile);
+                t.a = tile_plan_args(j->tile);
+                t.H = H;
+                t.nf = nf[k];
+                t.B = j->B64.get();
+                auto& cls = tiles[tile_plan_vy_tile(j->tile)];
+                cls.first.push_back(t);
+                cls.second.add((uint64_t)tile_plan_nitems(j->tile));
+            }
+            // shift-and-add chunks: chunk c of every job that has one; frame 0 of a
+            // job starts its sum and reference spectrum on its own
+            struct SaaClass {
+                std::vector<GroupSaa> tab;
+                std::array<Offsets, 5> off;
+                size_t lds_rows = 0, lds_cols = 0;
+            };
+            std::map<std::pair<int64_t, std::pair<int, int>>, SaaClass> saac; // (chunk index, shape)
+            std::vector<size_t> first_frame;
+            for (size_t k : R) {
+                GroupMember& m = M[k];
+                smip_job* j = m.j;
+                const smip_job_params& p = j->p;
+                if (!p.saa)
+                    continue;
+                const int64_t first = j->nframes == 0 ? 1 : 0;
+                if (first)
+                    first_frame.push_back(k);
+                const int64_t mm = nf[k] - first;
+                const int64_t chunk = saa_chunk(p.ncols, p.nrows, j->batch);
+                for (int64_t c = 0; c * chunk < mm; ++c) {
+                    SaaClass& sc = saac[{ c, { (int)p.ncols, (int)p.nrows } }];
+                    sc.lds_rows = std::max(sc.lds_rows, m.t.lds_rows64);
+                    sc.lds_cols = std::max(sc.lds_cols, m.t.lds_cols64);
+                    GroupSaa g {};
+                    const int64_t o = first + c * chunk;
+                    g.frames = m.base + (m.pos + (uint64_t)o) * m.fb;
+                    g.dtype = p.frame_dtype;
+                    g.X = (int32_t)p.ncols;
+                    g.Y = (int32_t)p.nrows;
+                    g.lx = m.t.lx;
+                    g.ly = m.t.ly;
+                    g.hx = g.X / 2 + 1;
+                    g.ppw = m.t.ppw64;
+                    g.cb = m.t.cb64;
+                    g.nblk = m.t.nblk64;
+                    g.nf = std::min(chunk, mm - c * chunk);
+                    g.parts = (int32_t)saa_add_parts(g.dtype, g.nf, g.X, g.Y);
+                    g.twx = m.t.twx64;
+                    g.twy = m.t.twy64;
+                    g.ref = j->ref_spec.get();
+                    g.G = j->swork.get();
+                    g.best = reinterpret_cast<RowBest*>(j->corr.get());
+                    g.shifts = j->shifts.get() + 2 * (j->nframes + (uint64_t)o);
+                    g.sum = j->sum.get();
+                    sc.tab.push_back(g);
+                    auto& of = sc.off;
+                    of[0].add((uint64_t)((g.Y / 2) / g.ppw) * g.nf);
+                    of[1].add((uint64_t)((g.hx + g.cb - 1) / g.cb) * g.nf);
+                    of[2].add((uint64_t)g.nblk * g.nf);
+                    of[3].add((uint64_t)g.nf);
+                    of[4].add(((uint64_t)g.X * g.Y + 255) / 256 * (uint64_t)g.parts);
+                }
+            }
+            // one upload for the round, on gs, once fs and ss are done with the slot
+            TableImage img;
+            std::vector<std::array<size_t, 4>> at_fwd;
+            for (auto& kv : fwdc)
+                at_fwd.push_back({ img.add(kv.second.tab.data(), kv.second.tab.size()),
+                    img.add(kv.second.rows.off.data(), kv.second.rows.off.size()),
+                    img.add(kv.second.cols.off.data(), kv.second.cols.off.size()),
+                    img.add(kv.second.pow.off.data(), kv.second.pow.off.size()) });
+            std::vector<std::pair<size_t, size_t>> at_tile;
+            for (auto& kv : tiles)
+                at_tile.push_back({ img.add(kv.second.first.data(), kv.second.first.size()),
+                    img.add(kv.second.second.off.data(), kv.second.second.off.size()) });
+            std::vector<std::pair<size_t, size_t>> at_saa;
+            for (auto& kv : saac) {
+                std::vector<uint32_t> offs;
+                for (int q = 0; q < 5; ++q)
+                    offs.insert(offs.end(), kv.second.off[q].off.begin(), kv.second.off[q].off.end());
+                at_saa.push_back({ img.add(kv.second.tab.data(), kv.second.tab.size()), img.add(offs.data(), offs.size()) });
+            }
+            const int slot = ring.cur;
+            if (slot_used[slot]) {
+                SMIP_HIP(hipStreamWaitEvent(gs, e_fs_done[slot], 0));
+                SMIP_HIP(hipStreamWaitEvent(gs, e_ss_done[slot], 0));
+            }
+            uint8_t* dt = ring.upload(img, gs);
+            SMIP_HIP(hipEventRecord(e_up, gs));
+            SMIP_HIP(hipStreamWaitEvent(fs, e_up, 0));
+            SMIP_HIP(hipStreamWaitEvent(ss, e_up, 0));
+            RoundEvents ev { timing_event(), timing_event(), timing_event(), timing_event(), timing_event(), R };
+            rounds.push_back(ev);
+            // transforms into spectrum half `half`, once round - 2's triple products have read it
+            if (tile_used[half])
+                SMIP_HIP(hipStreamWaitEvent(fs, e_tile[half], 0));
+            SMIP_HIP(hipEventRecord(ev.f0, fs));
+            size_t fi = 0;
+            for (auto& kv : fwdc) {
+                const auto& a = at_fwd[fi++];
+                const FwdClass& fc = kv.second;
+                launch_group_fwd(reinterpret_cast<const GroupFwd*>(dt + a[0]), reinterpret_cast<const uint32_t*>(dt + a[1]),
+                    reinterpret_cast<const uint32_t*>(dt + a[2]), reinterpret_cast<const uint32_t*>(dt + a[3]),
+                    (int)fc.tab.size(), fc.rows.total(), fc.cols.total(), fc.pow.total(), fc.lds_rows, fc.lds_cols, fs);
+            }
+            SMIP_HIP(hipEventRecord(e_fft, fs));
+            SMIP_HIP(hipEventRecord(e_fs_done[slot], fs));
+            // triple products
+            SMIP_HIP(hipStreamWaitEvent(gs, e_fft, 0));
+            SMIP_HIP(hipEventRecord(ev.t0, gs));
+            size_t ti = 0;
+            for (auto& kv : tiles) {
+                launch_group_tile(kv.first, reinterpret_cast<const GroupTile*>(dt + at_tile[ti].first),
+                    reinterpret_cast<const uint32_t*>(dt + at_tile[ti].second), (int)kv.second.first.size(),
+                    kv.second.second.total(), gs);
+                ++ti;
+            }
+            SMIP_HIP(hipEventRecord(ev.t1, gs));
+            SMIP_HIP(hipEventRecord(e_tile[half], gs));
+            tile_used[half] = true;
+            // shift-and-add
+            SMIP_HIP(hipEventRecord(ev.s0, ss));
+            for (size_t k : first_frame) { // sumarray = Re(frame 0); reference spectrum of frame 0
+                GroupMember& m = M[k];
+                smip_job* j = m.j;
+                const int64_t np = (int64_t)j->p.ncols * j->p.nrows;
+                const uint8_t* fr = m.base + m.pos * m.fb;
+                hipLaunchKernelGGL(k_frame_to_f64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ss, fr,
+                    j->p.frame_dtype, np, j->sum.get());
+                launch_rfft_f64(j->plan, fr, j->p.frame_dtype, 1, false, j->ref_spec.get(), j->swork.get(), ss);
+            }
+            size_t si = 0;
+            for (auto& kv : saac) { // chunk index ascending: each job's chunks in frame order
+                const SaaClass& sc = kv.second;
+                uint32_t nb[5];
+                for (int q = 0; q < 5; ++q)
+                    nb[q] = sc.off[q].total();
+                launch_group_saa(reinterpret_cast<const GroupSaa*>(dt + at_saa[si].first),
+                    reinterpret_cast<const uint32_t*>(dt + at_saa[si].second), (int)sc.tab.size(), nb, sc.lds_rows,
+                    sc.lds_cols, ss);
+                ++si;
+            }
+            SMIP_HIP(hipEventRecord(ev.s1, ss));
+            SMIP_HIP(hipEventRecord(e_ss_done[slot], ss));
+            slot_used[slot] = true;
+            for (size_t k : R) {
+                M[k].j->nframes += (uint64_t)nf[k];
+                M[k].pos += (uint64_t)nf[k];
+                M[k].j->tile_launches++;
+            }
+        }
+        for (hipStream_t st : { fs, ss, gs })
+            SMIP_HIP(hipStreamSynchronize(st));
+        // the round times, shared by the round's jobs
+        for (auto& r : rounds) {
+            float acc = 0.f, tl = 0.f, sa = 0.f;
+            SMIP_HIP(hipEventElapsedTime(&acc, r.f0, r.t1));
+            SMIP_HIP(hipEventElapsedTime(&tl, r.t0, r.t1));
+            SMIP_HIP(hipEventElapsedTime(&sa, r.s0, r.s1));
+            for (size_t k : r.members) {
+                M[k].j->ms[0] += acc;
+                M[k].j->ms[4] += tl;
+                M[k].j->ms[1] += sa;
+            }
+        }
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+} // namespace
+
+extern "C" SMIP_API int smip_job_add_frames_many(smip_job* const* jobs, const smip_frames* const* stacks,
+    const uint64_t* first, const uint64_t* count, uint64_t n)
+{
+    return guarded([&] {
+        require(n == 0 || (jobs && stacks && first && count), SMIP_ERR_INVALID_ARGUMENT, "null pointer");
+        if (n == 0)
+            return;
+        require(n <= 4096, SMIP_ERR_INVALID_ARGUMENT, "too many jobs in one group");
+        for (uint64_t k = 0; k < n; ++k) {
+            smip_job* j = jobs[k];
+            const smip_frames* f = stacks[k];
+            require(j && f, SMIP_ERR_INVALID_ARGUMENT, "null job or frame stack");
+            require(!j->finalized && !j->scaled, SMIP_ERR_LOGIC, "job already finalized");
+            require(j->staged == 0, SMIP_ERR_LOGIC, "frames pushed with smip_job_push_frames_host are pending");
+            require(f->dtype == j->p.frame_dtype && f->X == j->p.ncols && f->Y == j->p.nrows, SMIP_ERR_INVALID_ARGUMENT,
+                "frame stack does not match the job");
+            require(first[k] + count[k] <= f->count, SMIP_ERR_OUT_OF_RANGE, "frame range outside the stack");
+            require(j->ctx->device == jobs[0]->ctx->device && f->ctx->device == jobs[0]->ctx->device,
+                SMIP_ERR_INVALID_ARGUMENT, "jobs of one group must share a device");
+            for (uint64_t m = 0; m < k; ++m)
+                require(jobs[m] != j, SMIP_ERR_INVALID_ARGUMENT, "a job appears twice in the group");
+        }
+        ensure_device(jobs[0]->ctx);
+        // every job's earlier work is done: the grouped launches use its buffers from another stream
+        for (uint64_t k = 0; k < n; ++k)
+            for (hipStream_t st : { jobs[k]->ustream, jobs[k]->fstream, jobs[k]->sstream, jobs[k]->stream })
+                if (st)
+                    SMIP_HIP(hipStreamSynchronize(st));
+        std::vector<GroupMember> M;
+        for (uint64_t k = 0; k < n; ++k) {
+            smip_job* j = jobs[k];
+            if (count[k] == 0)
+                continue;
+            grow_shifts(j, j->nframes + count[k]);
+            const size_t fb = j->p.ncols * j->p.nrows * dtype_size(j->p.frame_dtype);
+            const uint8_t* base = static_cast<const uint8_t*>(stacks[k]->data) + first[k] * fb;
+            if (j->p.precision == SMIP_PRECISION_MIXED && fft_plan_pow2(j->plan)) {
+                GroupMember m;
+                m.j = j;
+                m.base = base;
+                m.total = count[k];
+                m.t = fft_tiling(j->plan);
+                m.fb = fb;
+                M.push_back(m);
+            } else {
+                job_process(j, base, count[k]); // its own launches, on its own streams
+            }
+        }
+        if (!M.empty())
+            run_group(M);
     });
 }
 

@@ -170,15 +170,6 @@ void launch_accum_plain_f32(const Desc& d, const LoopBounds& lb, const HalfPlane
 // ---------------------------------------------------------------------------
 // batched tile accumulation
 // ---------------------------------------------------------------------------
-struct TileArgs {
-    int32_t X, Y, pad, rs;
-    int64_t fs;
-    int32_t half_x, half_y; // ncols >> 1, nrows >> 1
-    int32_t min4, max4;     // valid vy range (loop bounds)
-    int32_t s1, s3;         // sizes[1], sizes[3]
-    int32_t bs1, bs2, bs3;  // base_sizes[1..3]
-};
-
 struct TilePlan {
     int vy_tile;
     int64_t nitems; // including padding items (cu < 0)
@@ -202,11 +193,10 @@ int tile_plan_tail_needed(const Desc& d, int nrows, int vy_tile)
 }
 
 template <int VYT>
-__global__ __launch_bounds__(64) void k_accum_tile(const int4* __restrict__ items, TileArgs a,
-    const float2* __restrict__ H, int64_t nf, double2* __restrict__ B)
+__device__ __forceinline__ void accum_tile_body(const int4 it, const TileArgs& a, const float2* __restrict__ H,
+    int64_t nf, double2* __restrict__ B)
 {
     __shared__ float2 strip[64 + VYT];
-    const int4 it = items[blockIdx.x];
     const int cu = it.x, vx = it.y, r0 = it.z, vy0 = it.w;
     if (cu < 0)
         return;
@@ -273,6 +263,52 @@ __global__ __launch_bounds__(64) void k_accum_tile(const int4* __restrict__ item
     }
 }
 
+template <int VYT>
+__global__ __launch_bounds__(64) void k_accum_tile(const int4* __restrict__ items, TileArgs a,
+    const float2* __restrict__ H, int64_t nf, double2* __restrict__ B)
+{
+    accum_tile_body<VYT>(items[blockIdx.x], a, H, nf, B);
+}
+
+// The tile kernel over the items of many jobs: a job's items occupy a range of
+// blocks starting at a multiple of 8, so each item runs on the XCD its position
+// in the job's own list gives it.
+template <int VYT>
+__global__ __launch_bounds__(64) void k_group_tile(const GroupTile* __restrict__ tab, const uint32_t* __restrict__ off,
+    int n)
+{
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupTile& J = tab[j];
+    accum_tile_body<VYT>(load_const(J.items, g - off[j]), J.a, global_ptr(J.H), J.nf, global_ptr(J.B));
+}
+
+void launch_group_tile(int vy_tile, const GroupTile* tab, const uint32_t* off, int n, uint32_t nblocks, hipStream_t s)
+{
+    if (nblocks == 0 || n <= 0)
+        return;
+    switch (vy_tile) {
+    case 9:
+        hipLaunchKernelGGL(k_group_tile<9>, dim3(nblocks), dim3(64), 0, s, tab, off, n);
+        break;
+    case 13:
+        hipLaunchKernelGGL(k_group_tile<13>, dim3(nblocks), dim3(64), 0, s, tab, off, n);
+        break;
+    case 17:
+        hipLaunchKernelGGL(k_group_tile<17>, dim3(nblocks), dim3(64), 0, s, tab, off, n);
+        break;
+    case 25:
+        hipLaunchKernelGGL(k_group_tile<25>, dim3(nblocks), dim3(64), 0, s, tab, off, n);
+        break;
+    case 33:
+        hipLaunchKernelGGL(k_group_tile<33>, dim3(nblocks), dim3(64), 0, s, tab, off, n);
+        break;
+    default:
+        fail(SMIP_ERR_LOGIC, "vy tile not instantiated");
+    }
+    SMIP_HIP(hipGetLastError());
+}
+
 static const int kVyTiles[] = { 9, 13, 17, 25, 33 };
 
 // Default vy tile: least modelled time, chunks x tile width / (the width's
@@ -307,6 +343,9 @@ int tile_default_vy(const LoopBounds& L, int X, int Y)
 
 int tile_plan_vy_tile(const TilePlan* tp) { return tp->vy_tile; }
 uint64_t tile_plan_cells(const TilePlan* tp) { return tp->cells; }
+const int4* tile_plan_items(const TilePlan* tp) { return tp->items; }
+int64_t tile_plan_nitems(const TilePlan* tp) { return tp->nitems; }
+const TileArgs& tile_plan_args(const TilePlan* tp) { return tp->args; }
 
 TilePlan* tile_plan_create(const Desc& d, const LoopBounds& L, const HalfPlane& hp, int vy_tile, hipStream_t stream)
 {

@@ -304,16 +304,15 @@ __device__ inline void gather_tile(int total, Load&& load, Store&& store)
 // ---------------------------------------------------------------------------
 // Row pass: real rows y0 = 2p, y1 = 2p+1 of frame f -> G[f][y][kx], kx in [0, X/2]
 // ---------------------------------------------------------------------------
+// row-pair block bx of frame f; smem holds 2 ppw X elements of V
 template <typename V, typename In>
-__global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ frames, int X, int Y, int lx, int ppw,
-    const V* __restrict__ tw, V* __restrict__ G, int hx, int absval)
+__device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int X, int Y, int lx, int ppw,
+    const V* __restrict__ tw, V* __restrict__ G, int hx, int absval, int bx, int64_t f, char* smem)
 {
     using S = typename scalar_of<V>::type;
-    extern __shared__ char smem[];
     V* A = reinterpret_cast<V*>(smem);
     V* B = A + ppw * X;
-    const int64_t f = blockIdx.y;
-    const int pair0 = blockIdx.x * ppw;
+    const int pair0 = bx * ppw;
     const In* fr = frames + f * (int64_t)X * Y;
     gather_tile<kRowPer, V>(
         ppw * X,
@@ -339,19 +338,29 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ fr
     }
 }
 
+template <typename V, typename In>
+__global__ __launch_bounds__(kThreads) void k_rows_fwd(const In* __restrict__ frames, int X, int Y, int lx, int ppw,
+    const V* __restrict__ tw, V* __restrict__ G, int hx, int absval)
+{
+    extern __shared__ char smem[];
+    rows_fwd_body<V, In>(frames, X, Y, lx, ppw, tw, G, hx, absval, blockIdx.x, blockIdx.y, smem);
+}
+
 // Column blocks of one row segment share 128-byte lines (a block holds 64 bytes
 // of a row at the default tile), and workgroups go to the 8 XCDs, each with its
 // own L2, round-robin in dispatch order.  The dispatch index is remapped so that
 // each XCD takes one contiguous run of (frame, column block) indices: neighbours
 // that share lines then share an L2.  A bijection on the first multiple of 8
 // indices, the identity on the rest.
+__device__ inline uint32_t xcd_remap(uint32_t d, uint32_t n)
+{
+    const uint32_t full = n & ~7u;
+    return d < full ? (d & 7u) * (full >> 3) + (d >> 3) : d;
+}
 __device__ inline void xcd_tile(int& bx, int64_t& f)
 {
     const uint32_t gx = gridDim.x, n = gx * gridDim.y;
-    uint32_t d = blockIdx.x + blockIdx.y * gx;
-    const uint32_t full = n & ~7u;
-    if (d < full)
-        d = (d & 7u) * (full >> 3) + (d >> 3);
+    const uint32_t d = xcd_remap(blockIdx.x + blockIdx.y * gx, n);
     bx = (int)(d % gx);
     f = (int64_t)(d / gx);
 }
@@ -363,15 +372,11 @@ __device__ inline void xcd_tile(int& bx, int64_t& f)
 //   MODE 1: natural r2c layout R[f][ky][kx]
 // ---------------------------------------------------------------------------
 template <typename V, int MODE, typename Out>
-__global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, int X, int Y, int ly, int hx, int cb,
-    const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp)
+__device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, int Y, int ly, int hx, int cb,
+    const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp, int bx, int64_t f, char* smem)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
-    extern __shared__ char smem[];
     V* A = reinterpret_cast<V*>(smem);
-    int bx;
-    int64_t f;
-    xcd_tile(bx, f);
     const int kx0 = bx * cb;
     gather_tile<kColPer<V>, V>(
         cb * Y,
@@ -414,12 +419,23 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, 
     }
 }
 
+template <typename V, int MODE, typename Out>
+__global__ __launch_bounds__(kThreads) void k_cols_fwd(const V* __restrict__ G, int X, int Y, int ly, int hx, int cb,
+    const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp)
+{
+    extern __shared__ char smem[];
+    int bx;
+    int64_t f;
+    xcd_tile(bx, f);
+    cols_fwd_body<V, MODE, Out>(G, X, Y, ly, hx, cb, tw, out, hp, bx, f, smem);
+}
+
 // P[c][r] += |H[f][c][pad + r]|^2 for f = 0..nf-1 in order (FP64), the power
 // spectrum of smip-cli/main.cpp:274-279,300-305 on the stored half plane.
 template <typename V>
-__global__ void k_power_acc(const V* __restrict__ H, int64_t nf, HalfPlane hp, double* __restrict__ P)
+__device__ __forceinline__ void power_body(const V* __restrict__ H, int64_t nf, const HalfPlane& hp,
+    double* __restrict__ P, int64_t i)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)hp.hx * hp.nrows;
     if (i >= n)
         return;
@@ -446,6 +462,12 @@ __global__ void k_power_acc(const V* __restrict__ H, int64_t nf, HalfPlane hp, d
         acc += re * re + im * im;
     }
     P[i] = acc;
+}
+
+template <typename V>
+__global__ void k_power_acc(const V* __restrict__ H, int64_t nf, HalfPlane hp, double* __restrict__ P)
+{
+    power_body<V>(H, nf, hp, P, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Column pass of the cross-correlation: C = conj(Rref) * R along ky, inverse
@@ -494,16 +516,12 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
 // element (as k_xcorr_cols_inv), and the inverse transform, written back over G.
 // The spectrum stays in LDS: one read and one write of G instead of two each,
 // with the same arithmetic in the same order as the two separate passes.
-__global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restrict__ Rref, int X, int Y, int ly,
-    int hx, int cb, const double2* __restrict__ tw, double2* __restrict__ G)
+__device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref, int X, int Y, int ly, int hx, int cb,
+    const double2* __restrict__ tw, double2* __restrict__ G, int bx, int64_t f, char* smem)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
     constexpr int kPer = kColPer<double2>;
-    extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
-    int bx;
-    int64_t f;
-    xcd_tile(bx, f);
     const int kx0 = bx * cb;
     gather_tile<kPer, double2>(
         cb * Y,
@@ -549,6 +567,16 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
         if (kx < hx)
             G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restrict__ Rref, int X, int Y, int ly,
+    int hx, int cb, const double2* __restrict__ tw, double2* __restrict__ G)
+{
+    extern __shared__ char smem[];
+    int bx;
+    int64_t f;
+    xcd_tile(bx, f);
+    xcorr_cols_body(Rref, X, Y, ly, hx, cb, tw, G, bx, f, smem);
 }
 
 // Row pass of a c2r transform, two rows at a time: z = IDFT(H0 + i H1) gives
@@ -608,16 +636,15 @@ __device__ inline void better_of(double& bv, int64_t& bi, double v, int64_t i)
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __restrict__ G, int X, int Y, int lx,
-    int ppw, int hx, const double2* __restrict__ tw, RowBest* __restrict__ best)
+// row-pair block bx of frame f; writes best[f nblk + bx]
+__device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__ G, int X, int Y, int lx, int ppw,
+    int hx, const double2* __restrict__ tw, RowBest* __restrict__ best, int bx, int64_t f, int nblk, char* smem)
 {
-    extern __shared__ char smem[];
     __shared__ double wv[kThreads / 64];
     __shared__ int64_t wi[kThreads / 64];
     double2* A = reinterpret_cast<double2*>(smem);
     double2* B = A + ppw * X;
-    const int64_t f = blockIdx.y;
-    const int pair0 = blockIdx.x * ppw;
+    const int pair0 = bx * ppw;
     gather_tile<kRowPer, double2>(
         ppw * X,
         [&](int t) {
@@ -646,7 +673,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __r
     // workgroup that holds it, from nothing elsewhere
     double bv = -INFINITY;
     int64_t bi = INT64_MAX;
-    if (blockIdx.x == 0) {
+    if (bx == 0) {
         bv = Z[0].x;
         bi = 0;
     }
@@ -671,19 +698,25 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __r
     if (threadIdx.x == 0) {
         for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
             better_of(bv, bi, wv[k], wi[k]);
-        best[f * gridDim.x + blockIdx.x] = RowBest { bv, bi };
+        best[f * nblk + bx] = RowBest { bv, bi };
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rows_c2r_argmax(const double2* __restrict__ G, int X, int Y, int lx,
+    int ppw, int hx, const double2* __restrict__ tw, RowBest* __restrict__ best)
+{
+    extern __shared__ char smem[];
+    rows_c2r_argmax_body(G, X, Y, lx, ppw, hx, tw, best, blockIdx.x, blockIdx.y, gridDim.x, smem);
 }
 
 // merges the workgroups' maxima of each frame in index order and applies the
 // signed wrap x >= (n+1)/2 -> x - n
-__global__ __launch_bounds__(64) void k_best_to_shift(const RowBest* __restrict__ best, int nblk, int64_t nf, int X,
-    int Y, int32_t* __restrict__ shifts)
+__device__ __forceinline__ void best_to_shift_body(const RowBest* __restrict__ best, int nblk, int64_t nf, int X, int Y,
+    int32_t* __restrict__ shifts, int64_t f)
 {
     // one wavefront per frame: each lane scans every 64th entry, then the lanes'
     // results merge; better_of keeps the larger value and, on ties, the lower
     // index, so the order of the merge does not change the result
-    const int64_t f = blockIdx.x;
     const int lane = threadIdx.x;
     if (f >= nf)
         return;
@@ -707,6 +740,158 @@ __global__ __launch_bounds__(64) void k_best_to_shift(const RowBest* __restrict_
         sy -= Y;
     shifts[2 * f] = sx;
     shifts[2 * f + 1] = sy;
+}
+
+__global__ __launch_bounds__(64) void k_best_to_shift(const RowBest* __restrict__ best, int nblk, int64_t nf, int X,
+    int Y, int32_t* __restrict__ shifts)
+{
+    best_to_shift_body(best, nblk, nf, X, Y, shifts, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped forms: the current batch of many jobs in one launch per stage (block
+// -> job through the offset table, then the single-job body)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_group_rows_fwd(const GroupFwd* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    extern __shared__ char smem[];
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupFwd& J = tab[j];
+    const uint32_t local = g - off[j];
+    const uint32_t nbx = (uint32_t)((J.Y / 2) / J.ppw);
+    const int bx = (int)(local % nbx);
+    const int64_t f = local / nbx;
+    switch (J.dtype) {
+    case SMIP_DTYPE_U8:
+        rows_fwd_body<float2, uint8_t>((const uint8_t*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw, global_ptr(J.twx),
+            global_ptr(J.G), J.hx, 0, bx, f, smem);
+        break;
+    case SMIP_DTYPE_U16:
+        rows_fwd_body<float2, uint16_t>((const uint16_t*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw, global_ptr(J.twx),
+            global_ptr(J.G), J.hx, 0, bx, f, smem);
+        break;
+    default:
+        rows_fwd_body<float2, float>((const float*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw, global_ptr(J.twx),
+            global_ptr(J.G), J.hx, 0, bx, f, smem);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_group_cols_fwd(const GroupFwd* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    extern __shared__ char smem[];
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = group_job(off, n, g);
+    const GroupFwd& J = tab[j];
+    const uint32_t local = g - off[j];
+    const uint32_t nbx = (uint32_t)((J.hx + J.cb - 1) / J.cb);
+    cols_fwd_body<float2, 0, float2>(global_ptr(J.G), J.X, J.Y, J.ly, J.hx, J.cb, global_ptr(J.twy), global_ptr(J.H),
+        J.hp, (int)(local % nbx), (int64_t)(local / nbx), smem);
+}
+
+__global__ void k_group_power(const GroupFwd* __restrict__ tab, const uint32_t* __restrict__ off, int n)
+{
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupFwd& J = tab[j];
+    power_body<float2>(global_ptr(J.H), J.nf, J.hp, global_ptr(J.P), (int64_t)(g - off[j]) * blockDim.x + threadIdx.x);
+}
+
+void launch_group_fwd(const GroupFwd* tab, const uint32_t* off_rows, const uint32_t* off_cols,
+    const uint32_t* off_pow, int n, uint32_t nb_rows, uint32_t nb_cols, uint32_t nb_pow, size_t lds_rows,
+    size_t lds_cols, hipStream_t s)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_group_rows_fwd, dim3(nb_rows), dim3(kThreads), lds_rows, s, tab, off_rows, n);
+    SMIP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_group_cols_fwd, dim3(nb_cols), dim3(kThreads), lds_cols, s, tab, off_cols, n);
+    SMIP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_group_power, dim3(nb_pow), dim3(256), 0, s, tab, off_pow, n);
+    SMIP_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kThreads) void k_group_saa_rows(const GroupSaa* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    extern __shared__ char smem[];
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupSaa& J = tab[j];
+    const uint32_t local = g - off[j];
+    const uint32_t nbx = (uint32_t)((J.Y / 2) / J.ppw);
+    const int bx = (int)(local % nbx);
+    const int64_t f = local / nbx;
+    switch (J.dtype) {
+    case SMIP_DTYPE_U8:
+        rows_fwd_body<double2, uint8_t>((const uint8_t*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw, global_ptr(J.twx),
+            global_ptr(J.G), J.hx, 1, bx, f, smem);
+        break;
+    case SMIP_DTYPE_U16:
+        rows_fwd_body<double2, uint16_t>((const uint16_t*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw,
+            global_ptr(J.twx), global_ptr(J.G), J.hx, 1, bx, f, smem);
+        break;
+    default:
+        rows_fwd_body<double2, float>((const float*)global_ptr(J.frames), J.X, J.Y, J.lx, J.ppw, global_ptr(J.twx),
+            global_ptr(J.G), J.hx, 1, bx, f, smem);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_group_saa_cols(const GroupSaa* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    extern __shared__ char smem[];
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = group_job(off, n, g);
+    const GroupSaa& J = tab[j];
+    const uint32_t local = g - off[j];
+    const uint32_t nbx = (uint32_t)((J.hx + J.cb - 1) / J.cb);
+    xcorr_cols_body(global_ptr(J.ref), J.X, J.Y, J.ly, J.hx, J.cb, global_ptr(J.twy), global_ptr(J.G),
+        (int)(local % nbx), (int64_t)(local / nbx), smem);
+}
+
+__global__ __launch_bounds__(kThreads) void k_group_saa_c2r(const GroupSaa* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    extern __shared__ char smem[];
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupSaa& J = tab[j];
+    const uint32_t local = g - off[j];
+    rows_c2r_argmax_body(global_ptr(J.G), J.X, J.Y, J.lx, J.ppw, J.hx, global_ptr(J.twx), global_ptr(J.best),
+        (int)(local % (uint32_t)J.nblk), (int64_t)(local / (uint32_t)J.nblk), J.nblk, smem);
+}
+
+__global__ __launch_bounds__(64) void k_group_saa_merge(const GroupSaa* __restrict__ tab,
+    const uint32_t* __restrict__ off, int n)
+{
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupSaa& J = tab[j];
+    best_to_shift_body(global_ptr(J.best), J.nblk, J.nf, J.X, J.Y, global_ptr(J.shifts), (int64_t)(g - off[j]));
+}
+
+void launch_group_saa(const GroupSaa* tab, const uint32_t* off, int n, const uint32_t nb[5], size_t lds_rows,
+    size_t lds_cols, hipStream_t s)
+{
+    if (n <= 0)
+        return;
+    const uint32_t* o = off;
+    hipLaunchKernelGGL(k_group_saa_rows, dim3(nb[0]), dim3(kThreads), lds_rows, s, tab, o, n);
+    SMIP_HIP(hipGetLastError());
+    o += n + 1;
+    hipLaunchKernelGGL(k_group_saa_cols, dim3(nb[1]), dim3(kThreads), lds_cols, s, tab, o, n);
+    SMIP_HIP(hipGetLastError());
+    o += n + 1;
+    hipLaunchKernelGGL(k_group_saa_c2r, dim3(nb[2]), dim3(kThreads), lds_rows, s, tab, o, n);
+    SMIP_HIP(hipGetLastError());
+    o += n + 1;
+    hipLaunchKernelGGL(k_group_saa_merge, dim3(nb[3]), dim3(64), 0, s, tab, o, n);
+    SMIP_HIP(hipGetLastError());
+    o += n + 1;
+    launch_group_saa_add(tab, o, n, nb[4], s);
 }
 
 // Complex 2-D transform passes (in place), used by the synthesis.
@@ -917,6 +1102,32 @@ void launch_best_to_shift(const RowBest* best, int nblk, int64_t nf, int X, int 
 int fft_shift_best_entries(const FftPlan* p)
 {
     return p->pow2 ? p->Y / 2 : mixed_shift_blocks(p);
+}
+
+bool fft_plan_pow2(const FftPlan* p) { return p->pow2; }
+
+FftTiling fft_tiling(const FftPlan* p)
+{
+    if (!p->pow2)
+        fail(SMIP_ERR_LOGIC, "grouped transforms take power-of-two plans");
+    FftTiling t;
+    const int X = p->X, Y = p->Y;
+    t.lx = p->lx;
+    t.ly = p->ly;
+    t.twx32 = p->twx32;
+    t.twy32 = p->twy32;
+    t.twx64 = p->twx64;
+    t.twy64 = p->twy64;
+    t.ppw32 = rows_per_wg<float2>(X, Y / 2);
+    t.cb32 = cols_per_wg<float2>(Y);
+    t.ppw64 = rows_per_wg<double2>(X, Y / 2);
+    t.cb64 = cols_per_wg<double2>(Y);
+    t.nblk64 = (Y / 2) / t.ppw64;
+    t.lds_rows32 = 2 * sizeof(float2) * t.ppw32 * X;
+    t.lds_cols32 = sizeof(float2) * t.cb32 * Y;
+    t.lds_rows64 = 2 * sizeof(double2) * t.ppw64 * X;
+    t.lds_cols64 = sizeof(double2) * t.cb64 * Y;
+    return t;
 }
 
 void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* spec, int64_t nf, double* corr,

@@ -11,6 +11,7 @@
 //  k_synth_prepare result = sqrt(Re P) * phase * Hann(2p) (smip-cli/main.cpp:329-338,
 //                  include/window_function.h:69-84)
 //  k_absmax_*      result /= max |result| (smip-cli/main.cpp:371-376)
+#include <algorithm>
 #include <cmath>
 
 #include "smip_complex.h"
@@ -98,10 +99,9 @@ __device__ inline double shifted_abs(const In* __restrict__ frames, int64_t f, i
 }
 
 template <typename In>
-__global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
-    const int32_t* __restrict__ shifts, double* __restrict__ sum)
+__device__ __forceinline__ void saa_add_body(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
+    const int32_t* __restrict__ shifts, double* __restrict__ sum, int64_t p)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int64_t)X * Y)
         return;
     const int32_t col = (int32_t)(p % X), row = (int32_t)(p / X);
@@ -122,6 +122,95 @@ __global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, 
     sum[p] = acc;
 }
 
+template <typename In>
+__global__ void k_saa_add(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
+    const int32_t* __restrict__ shifts, double* __restrict__ sum)
+{
+    saa_add_body(frames, nf, X, Y, shifts, sum, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Integer samples: |x| as a double is an integer, and the per-pixel sums stay far
+// below 2^53, so every partial sum is exact and their order cannot change a bit.
+// The frames are split into `parts` slices, one per grid row, each summed by its
+// own threads and added to the pixel with an FP64 atomic: sixteen times the
+// threads of the frame-ordered form on small frames, where that form leaves most
+// CUs idle.
+template <typename In>
+__device__ __forceinline__ void saa_add_int_body(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
+    const int32_t* __restrict__ shifts, double* __restrict__ sum, int64_t p, unsigned slice, unsigned parts)
+{
+    if (p >= (int64_t)X * Y)
+        return;
+    const int32_t col = (int32_t)(p % X), row = (int32_t)(p / X);
+    const int64_t per = (nf + parts - 1) / parts;
+    const int64_t f0 = (int64_t)slice * per, f1 = min(nf, f0 + per);
+    double acc = 0.0;
+    int64_t f = f0;
+    for (; f + 8 <= f1; f += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            v[k] = shifted_abs(frames, f + k, X, Y, col, row, shifts);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            acc += v[k];
+    }
+    for (; f < f1; ++f)
+        acc += shifted_abs(frames, f, X, Y, col, row, shifts);
+    if (f1 > f0)
+        atomicAdd(sum + p, acc);
+}
+
+template <typename In>
+__global__ void k_saa_add_int(const In* __restrict__ frames, int64_t nf, int32_t X, int32_t Y,
+    const int32_t* __restrict__ shifts, double* __restrict__ sum)
+{
+    saa_add_int_body(frames, nf, X, Y, shifts, sum, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
+        gridDim.y);
+}
+
+unsigned saa_add_parts(int dtype, int64_t nf, int32_t ncols, int32_t nrows)
+{
+    if (dtype != SMIP_DTYPE_U8 && dtype != SMIP_DTYPE_U16)
+        return 1;
+    // about 2048 workgroups in all, 8 frames or more per slice
+    const int64_t blocks = ((int64_t)ncols * nrows + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>({ 16, nf / 8, (2048 + blocks - 1) / blocks }));
+}
+
+// The shift-and-add of many jobs' chunks: per job ceil(N^2 / 256) blocks times
+// its slices; float frames keep the frame-ordered sum (one slice).
+__global__ void k_group_saa_add(const GroupSaa* __restrict__ tab, const uint32_t* __restrict__ off, int n)
+{
+    const uint32_t g = blockIdx.x;
+    const int j = group_job(off, n, g);
+    const GroupSaa& J = tab[j];
+    const uint32_t local = g - off[j];
+    const uint32_t pb = (uint32_t)(((int64_t)J.X * J.Y + 255) / 256);
+    const int64_t p = (int64_t)(local % pb) * blockDim.x + threadIdx.x;
+    const unsigned slice = local / pb;
+    switch (J.dtype) {
+    case SMIP_DTYPE_U8:
+        saa_add_int_body((const uint8_t*)global_ptr(J.frames), J.nf, J.X, J.Y, global_ptr(J.shifts), global_ptr(J.sum),
+            p, slice, (unsigned)J.parts);
+        break;
+    case SMIP_DTYPE_U16:
+        saa_add_int_body((const uint16_t*)global_ptr(J.frames), J.nf, J.X, J.Y, global_ptr(J.shifts),
+            global_ptr(J.sum), p, slice, (unsigned)J.parts);
+        break;
+    default:
+        saa_add_body((const float*)global_ptr(J.frames), J.nf, J.X, J.Y, global_ptr(J.shifts), global_ptr(J.sum), p);
+    }
+}
+
+void launch_group_saa_add(const GroupSaa* tab, const uint32_t* off, int n, uint32_t nblocks, hipStream_t s)
+{
+    if (n <= 0 || nblocks == 0)
+        return;
+    hipLaunchKernelGGL(k_group_saa_add, dim3(nblocks), dim3(256), 0, s, tab, off, n);
+    SMIP_HIP(hipGetLastError());
+}
+
 void launch_saa_add(const void* frames, int dtype, int64_t nf, int32_t ncols, int32_t nrows, const int32_t* shifts,
     double* sum, hipStream_t s)
 {
@@ -129,14 +218,15 @@ void launch_saa_add(const void* frames, int dtype, int64_t nf, int32_t ncols, in
         return;
     const int64_t n = (int64_t)ncols * nrows;
     dim3 grid((unsigned)((n + 255) / 256));
+    const unsigned parts = saa_add_parts(dtype, nf, ncols, nrows);
     switch (dtype) {
     case SMIP_DTYPE_U8:
-        hipLaunchKernelGGL(k_saa_add<uint8_t>, grid, dim3(256), 0, s, (const uint8_t*)frames, nf, ncols, nrows,
-            shifts, sum);
+        hipLaunchKernelGGL(k_saa_add_int<uint8_t>, dim3(grid.x, parts), dim3(256), 0, s, (const uint8_t*)frames, nf,
+            ncols, nrows, shifts, sum);
         break;
     case SMIP_DTYPE_U16:
-        hipLaunchKernelGGL(k_saa_add<uint16_t>, grid, dim3(256), 0, s, (const uint16_t*)frames, nf, ncols, nrows,
-            shifts, sum);
+        hipLaunchKernelGGL(k_saa_add_int<uint16_t>, dim3(grid.x, parts), dim3(256), 0, s, (const uint16_t*)frames, nf,
+            ncols, nrows, shifts, sum);
         break;
     case SMIP_DTYPE_F32:
         hipLaunchKernelGGL(k_saa_add<float>, grid, dim3(256), 0, s, (const float*)frames, nf, ncols, nrows, shifts,

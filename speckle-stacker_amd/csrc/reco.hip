@@ -708,7 +708,8 @@ __global__ void k_reco_map_many(const RecoJob* __restrict__ jobs, int64_t win)
     const int64_t s0 = win * kWindow;
     if (s0 >= a.nvisits)
         return;
-    reco_map_body(blockIdx.x, J.visits, J.first_pos, a, s0, min(s0 + kWindow, a.nvisits), win, J.map);
+    reco_map_body(blockIdx.x, global_ptr(J.visits), global_ptr(J.first_pos), a, s0, min(s0 + kWindow, a.nvisits), win,
+        global_ptr(J.map));
 }
 
 __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* __restrict__ jobs, int64_t win)
@@ -718,8 +719,9 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* 
     const int64_t s0 = win * kWindow;
     if (s0 + (int64_t)blockIdx.x >= a.nvisits)
         return;
-    reco_bulk_body(blockIdx.x, a, J.visits, s0, win, J.map, J.flags, J.phases, J.B, J.is_f32, J.S, J.N, J.E, J.cnt,
-        J.rec, J.cap, J.status);
+    reco_bulk_body(blockIdx.x, a, global_ptr(J.visits), s0, win, global_ptr(J.map), global_ptr(J.flags),
+        global_ptr(J.phases), global_ptr(J.B), J.is_f32, global_ptr(J.S), global_ptr(J.N), global_ptr(J.E),
+        global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
 }
 
 __global__ __launch_bounds__(64) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
@@ -729,8 +731,9 @@ __global__ __launch_bounds__(64) void k_reco_fixup_many(const RecoJob* __restric
     const int64_t s0 = win * kWindow;
     if (s0 >= a.nvisits)
         return;
-    reco_fixup_body(a, J.visits, J.first_pos, s0, min(s0 + kWindow, a.nvisits), J.S, J.N, J.E, J.cnt, J.rec, J.cap,
-        J.flags, J.phases, J.cons, J.status, win, J.map);
+    reco_fixup_body(a, global_ptr(J.visits), global_ptr(J.first_pos), s0, min(s0 + kWindow, a.nvisits), global_ptr(J.S),
+        global_ptr(J.N), global_ptr(J.E), global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.flags),
+        global_ptr(J.phases), global_ptr(J.cons), global_ptr(J.status), win, global_ptr(J.map));
 }
 
 // ---------------------------------------------------------------------------

@@ -252,6 +252,16 @@ void launch_accum_plain_f64(const Desc& d, const LoopBounds& lb, const HalfPlane
 void launch_accum_plain_f32(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, const double2* H,
     float2* B, hipStream_t s);
 
+// Arguments of the batched tile kernel, shared by the items of one job.
+struct TileArgs {
+    int32_t X, Y, pad, rs;
+    int64_t fs;
+    int32_t half_x, half_y; // ncols >> 1, nrows >> 1
+    int32_t min4, max4;     // valid vy range (loop bounds)
+    int32_t s1, s3;         // sizes[1], sizes[3]
+    int32_t bs1, bs2, bs3;  // base_sizes[1..3]
+};
+
 struct TilePlan; // work-item table of the batched accumulate
 TilePlan* tile_plan_create(const Desc& d, const LoopBounds& lb, const HalfPlane& hp, int vy_tile, hipStream_t stream);
 void tile_plan_destroy(TilePlan*);
@@ -262,6 +272,111 @@ int tile_plan_tail_needed(const Desc& d, int nrows, int vy_tile);
 void launch_accum_tile(const TilePlan* tp, const HalfPlane& hp, const float2* H, int64_t nf, double2* B64,
     hipStream_t s);
 uint64_t tile_plan_cells(const TilePlan* tp); // cells computed per frame
+const int4* tile_plan_items(const TilePlan* tp);  // device item table
+int64_t tile_plan_nitems(const TilePlan* tp);     // a multiple of 8
+const TileArgs& tile_plan_args(const TilePlan* tp);
+
+// ---------------------------------------------------------------------------
+// Grouped launches (smip_job_add_frames_many): one launch per stage for the
+// current batch of many jobs.  Block k of a grouped launch belongs to the job j
+// with off[j] <= k < off[j + 1] (off[0] = 0, off[n] = the launch's blocks); the
+// block then runs the single-job kernel's body for its local index.
+// ---------------------------------------------------------------------------
+// A device pointer read from a table in device memory, as a pointer into global
+// memory.  A generic pointer loaded from memory makes every access through it a
+// flat instruction, which also counts against the LDS wait counter, so each
+// wait then covers LDS and memory together, and the operands of a uniform load
+// land in vector registers.  The field is read as an integer because a cast of
+// the loaded generic pointer to global and back is folded away.
+template <class T>
+__device__ __forceinline__ T* global_ptr(T* const& field)
+{
+    const uint64_t v = *reinterpret_cast<const uint64_t*>(&field);
+    return (T*)(__attribute__((address_space(1))) T*)v;
+}
+
+// Element i of a table that no kernel writes, as a scalar (constant address
+// space) load when i is uniform.
+template <class T>
+__device__ __forceinline__ T load_const(const T* const& field, int64_t i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t v = *reinterpret_cast<const uint64_t*>(&field);
+    return ((const __attribute__((address_space(4))) T*)v)[i];
+#else
+    return field[i]; // the host pass only parses device functions
+#endif
+}
+
+__device__ inline int group_job(const uint32_t* __restrict__ off, int n, uint32_t g)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// one job's forward FP32 transform, |F|^2 and tile inputs for one batch
+struct GroupFwd {
+    const void* frames;
+    int32_t dtype, X, Y, lx, ly, hx, ppw, cb;
+    int64_t nf;
+    const float2* twx;
+    const float2* twy;
+    float2* G;
+    float2* H;
+    HalfPlane hp;
+    double* P;
+};
+struct GroupTile {
+    const int4* items;
+    TileArgs a;
+    const float2* H;
+    int64_t nf;
+    double2* B;
+};
+// one job's shift-and-add of one chunk of frames
+struct GroupSaa {
+    const void* frames;
+    int32_t dtype, X, Y, lx, ly, hx, ppw, cb, nblk, parts;
+    int64_t nf;
+    const double2* twx;
+    const double2* twy;
+    const double2* ref;
+    double2* G;
+    RowBest* best;
+    int32_t* shifts;
+    double* sum;
+};
+// Row and column tiling of the power-of-two transforms (the grouped launches use
+// the single-job launches' shapes).
+struct FftTiling {
+    int lx, ly; // log2 of the lengths
+    int ppw32, cb32, ppw64, cb64, nblk64;
+    size_t lds_rows32, lds_cols32, lds_rows64, lds_cols64;
+    const float2* twx32;
+    const float2* twy32;
+    const double2* twx64;
+    const double2* twy64;
+};
+FftTiling fft_tiling(const FftPlan* p);
+bool fft_plan_pow2(const FftPlan* p);
+// frame slices of the integer shift-and-add (1 for float frames)
+unsigned saa_add_parts(int dtype, int64_t nf, int32_t ncols, int32_t nrows);
+void launch_group_fwd(const GroupFwd* tab, const uint32_t* off_rows, const uint32_t* off_cols,
+    const uint32_t* off_pow, int n, uint32_t nb_rows, uint32_t nb_cols, uint32_t nb_pow, size_t lds_rows,
+    size_t lds_cols, hipStream_t s);
+void launch_group_tile(int vy_tile, const GroupTile* tab, const uint32_t* off, int n, uint32_t nblocks, hipStream_t s);
+// the five shift-and-add stages of one chunk index; off holds five (n + 1) arrays
+// (rows, columns, c2r, merge, add)
+void launch_group_saa(const GroupSaa* tab, const uint32_t* off, int n, const uint32_t nb[5], size_t lds_rows,
+    size_t lds_cols, hipStream_t s);
+void launch_group_saa_add(const GroupSaa* tab, const uint32_t* off, int n, uint32_t nblocks, hipStream_t s);
 
 void launch_scale_c64(double2* B, uint64_t n, double re, double im, int op, hipStream_t s);
 void launch_scale_c32(float2* B, uint64_t n, float re, float im, int op, hipStream_t s);

@@ -131,6 +131,7 @@ _SIGS = {
     "smip_job_create": (C.c_int, [_P, _P, _P]),
     "smip_job_destroy": (C.c_int, [_P]),
     "smip_job_add_frames": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
+    "smip_job_add_frames_many": (C.c_int, [_P, _P, _P, _P, C.c_uint64]),
     "smip_job_add_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_push_frames_host": (C.c_int, [_P, _P, C.c_uint64]),
     "smip_job_finalize": (C.c_int, [_P]),
@@ -452,6 +453,19 @@ def finalize_many(jobs: list) -> None:
     arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
     st = (C.c_int32 * len(jobs))()
     _check(lib().smip_job_finalize_many(arr, len(jobs), st))
+
+
+def add_frames_many(jobs: list, stacks: list, firsts=None, counts=None) -> None:
+    """Job.add_frames for several jobs at once: one launch per stage for the current
+    batch of every job (smip_job_add_frames_many)."""
+    n = len(jobs)
+    firsts = [0] * n if firsts is None else list(firsts)
+    counts = [s.shape[0] - f for s, f in zip(stacks, firsts)] if counts is None else list(counts)
+    ja = (C.c_void_p * n)(*[j.h for j in jobs])
+    sa = (C.c_void_p * n)(*[s.h for s in stacks])
+    fa = (C.c_uint64 * n)(*firsts)
+    ca = (C.c_uint64 * n)(*counts)
+    _check(lib().smip_job_add_frames_many(ja, sa, fa, ca, n))
 
 
 def resolve_params(params: JobParams) -> JobParams:

@@ -310,7 +310,7 @@ def test_server_errors(ctx):
     with pytest.raises(S.SmipError) as e:
         srv.wait(jid + 1000)
     assert e.value.code == S.ERR_INVALID_ARGUMENT
-    bad = S.job_params(48, 48, 16, 16, fr.dtype)  # not a power of two
+    bad = S.job_params(4096, 48, 16, 16, fr.dtype)  # a side beyond the 2048 the transforms hold
     with pytest.raises(S.SmipError):
         srv.submit(bad)
     assert srv.wait(jid, 60.0) == S.JOB_DONE
@@ -405,3 +405,49 @@ def test_finalize_many_rules(ctx):
     assert e.value.code == S.ERR_LOGIC
     a.close()
     b.close()
+
+
+def test_add_frames_many_matches_single_jobs(ctx):
+    """smip_job_add_frames_many: one launch per stage for the batches of many jobs of
+    uneven frame side, frame count, depth, sample type and tile width -- plus an
+    FP64 job and a 96x64 job, which take their own launches -- gives each job
+    exactly what smip_job_add_frames gives it, also when the frames arrive over
+    several grouped calls."""
+    specs = [("binary128", 64, 150, 16, 0, S.PRECISION_MIXED), ("triple256", 128, 90, 24, 9, S.PRECISION_MIXED),
+             ("extended512", 64, 40, 33, 0, S.PRECISION_MIXED), ("triple256", 256, 70, 48, 25, S.PRECISION_MIXED),
+             ("binary128", 32, 13, 12, 17, S.PRECISION_MIXED), ("extended512", 128, 30, 20, 13, S.PRECISION_MIXED),
+             ("triple256", 64, 25, 16, 0, S.PRECISION_FP64)]
+    stacks, params = [], []
+    for k, (kind, N, n, b, vy, prec) in enumerate(specs):
+        fr = _frames(kind, N, n, seed_offset=k)
+        stacks.append(S.Frames(ctx, fr))
+        params.append(S.job_params(N, N, b, 2 * b, fr.dtype, prec, frames_per_batch=32, vy_tile=vy))
+    odd = np.random.default_rng(5).integers(0, 255, size=(20, 64, 96), dtype=np.uint8)
+    stacks.append(S.Frames(ctx, odd))
+    params.append(S.job_params(96, 64, 12, 20, np.uint8, S.PRECISION_MIXED, frames_per_batch=8))
+
+    # the pieces each call gets: whole stacks, then uneven pieces with some jobs idle
+    rng = np.random.default_rng(11)
+    pos = [0] * len(stacks)
+    rounds = []
+    while any(p < s.shape[0] for p, s in zip(pos, stacks)):
+        cnt = [int(min(s.shape[0] - p, rng.integers(0, 70))) for p, s in zip(pos, stacks)]
+        rounds.append(list(zip(pos, cnt)))
+        pos = [p + c for p, c in zip(pos, cnt)]
+    for split in ([[(0, s.shape[0]) for s in stacks]], rounds):
+        want = []
+        for k, (st, p) in enumerate(zip(stacks, params)):
+            j = S.Job(ctx, p)
+            for piece in split:
+                if piece[k][1]:
+                    j.add_frames(st, piece[k][0], piece[k][1])
+            j.finalize()
+            want.append(_outputs(j))
+            j.close()
+        jobs = [S.Job(ctx, p) for p in params]
+        for piece in split:
+            S.add_frames_many(jobs, stacks, [f for f, _ in piece], [c for _, c in piece])
+        S.finalize_many(jobs)
+        for j, w in zip(jobs, want):
+            _assert_identical(_outputs(j), w)
+            j.close()
