@@ -2662,7 +2662,7 @@ int smip_amd::job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* s
                 const smip_job_params& p = j->p;
                 j->reco_calls = (uint64_t)calls[k];
                 j->ms[2] += reco_ms; // the group's recovery time, shared
-                note(k, st[k], "reconstruct_phases: bispectrum element address out of bounds");
+                note(k, st[k], reco_error_text(st[k]));
                 if (st[k] != SMIP_OK)
                     continue; // left unfinalized, as smip_job_finalize leaves a job that throws
                 SMIP_HIP(hipStreamWaitEvent(j->stream, t1, 0));

@@ -17,9 +17,12 @@
 //    u or v is a point visited earlier in the same window (it may get flagged
 //    before this visit): beta-hat, the pre-window flags and phases, and the
 //    window slots of the in-window points.  At most 2(K-1) records per visit.
-//  * k_reco_fixup (one wavefront) walks the window's visits in order; each step
-//    adds the pending pairs whose points are flagged by now, reading only LDS,
-//    and finalises the point exactly as calc_phase does.
+//  * k_reco_fixup (one workgroup) walks the window's visits in order on one
+//    wavefront; each step adds the pending pairs whose points are flagged by
+//    now and finalises the point exactly as calc_phase does.  Helper
+//    wavefronts run a few visits ahead and pre-sum every pending pair whose
+//    outcome is already decided, so the step itself handles only the pairs
+//    with a point flagged in the last few visits (reco_fixup_body).
 //
 // Eligibility is evaluated exactly, so the flag mask is bit-exact; only the
 // order of the FP64 summation of the mean differs from the reference's raster
@@ -42,7 +45,6 @@ namespace smip_amd {
 constexpr double kEps = 2.220446049250313e-16; // constants::c_epsilon<double>
 constexpr int kWindow = 256;                   // visits per window
 constexpr int kBulkThreads = 1024;            // lanes per visit in the bulk pass (latency-bound loads)
-constexpr int kFixBuf = 512; // pending records staged in LDS per chunk
 
 struct RecoArgs {
     Desc d;
@@ -148,6 +150,16 @@ template <int CTRL, int ROWS = 0xf>
 __device__ inline int dpp_i(int v)
 {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+// the same where every lane reads an active lane of its own row (quad
+// permutes and mirrors): no old value to carry
+template <int CTRL>
+__device__ inline double dpp_row_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 template <int CTRL, int ROWS = 0xf>
 __device__ inline double dpp_d(double v)
@@ -417,225 +429,448 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
     }
 }
 
+// Step scalars of one visit, 32 bytes: two 16-byte LDS loads.
 struct StepInfo {
     int32_t n0;  // pre-window eligible count, -1: the visit does nothing
     int32_t fp;  // window slot of the first visit of this point
     int32_t cnt; // pending records
-    int32_t off; // offset of the records within the chunk buffer
-    int32_t pw;  // PhaseMap position of the point
     int32_t err; // an eligible pre-window lookup was out of range
-    int32_t rev; // not the first visit of the point in this window
-    int32_t pad;
     double2 S;   // pre-window sum
 };
 
-constexpr double kNotYet = 2.0;  // ppos marker: slot not flagged in this window
-constexpr int kChunk = 8;        // visits per staged chunk
-constexpr int kChunkRecs = 512;  // records per chunk buffer
-constexpr int kParts = sizeof(PendingRec) / sizeof(int4);
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x3 __attribute__((ext_vector_type(3)));
 
-// Stage chunk c's pending records into the LDS buffer dst with direct
-// global->LDS loads (global_load_lds_dwordx4: lane i's 16 bytes land at
-// dst + 16 i).  Piece p = u*64 + lane belongs to record t = p / kParts of the
-// chunk, owned by the last visit whose chunk-local offset is <= t.  The loads
-// complete asynchronously; the caller waits (vmcnt) before reading dst.
-__device__ __forceinline__ void fix_stage_chunk(int4* dst, const int4* __restrict__ src4, const StepInfo* info, int c,
-    int nsteps, int pieces, int cap, int lane)
+constexpr double kNotYet = 2.0;           // ppos marker: slot not flagged in this window
+constexpr int kFixWaves = 8;              // the chain wave and seven helpers
+constexpr int kFixThreads = 64 * kFixWaves;
+constexpr int kHelpers = kFixWaves - 1;
+constexpr int kLead = 8;                  // visits between a helper's snapshot and its visit
+constexpr int kRing = 16;                 // near-record slots, more than kLead
+constexpr int kNearCap = 32;              // near records per visit; more: the chain takes every record
+constexpr int32_t kNever = 0x7fffffff;    // ftime of a slot not flagged in this window
+constexpr uint32_t kSpinLimit = 1u << 24; // polls (each after a short sleep) before a wait counts as stuck
+
+// Sum over the first 16 lanes (row 0) with four DPP levels; the total ends in
+// every lane of the row and is read from lane 0.  Used when only lanes 0-15
+// hold terms.
+__device__ inline void row_sum3(int& n, double& re, double& im)
 {
-    const int lq0 = c * kChunk, lq1 = min(nsteps, lq0 + kChunk);
-    int offs[kChunk];
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k)
-        offs[k] = lq0 + k < lq1 ? info[lq0 + k].off : (1 << 30);
-    for (int u0 = 0; u0 < pieces; u0 += 64) {
-        const int pc = u0 + lane;
-        const int t = pc / kParts, part = pc - t * kParts;
-        int k = 0, o = 0;
-#pragma unroll
-        for (int kk = 0; kk < kChunk; ++kk)
-            if (offs[kk] <= t) {
-                k = kk;
-                o = offs[kk];
-            }
-        if (pc < pieces)
-            __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void*)(src4
-                    + ((int64_t)(lq0 + k) * cap + (t - o)) * kParts + part),
-                (__attribute__((address_space(3))) void*)(dst + u0), 16, 0, 0);
+#define SMIP_LEVEL(CTRL)                                                                                    \
+    {                                                                                                       \
+        const int tn = dpp_i<CTRL, 0xf>(n);                                                                 \
+        const double tr = dpp_d<CTRL, 0xf>(re), ti = dpp_d<CTRL, 0xf>(im);                                  \
+        n += tn;                                                                                            \
+        re += tr;                                                                                           \
+        im += ti;                                                                                           \
+    }
+    SMIP_LEVEL(0xB1)  // quad_perm [1,0,3,2]
+    SMIP_LEVEL(0x4E)  // quad_perm [2,3,0,1]
+    SMIP_LEVEL(0x141) // row_half_mirror
+    SMIP_LEVEL(0x140) // row_mirror
+#undef SMIP_LEVEL
+    n = __builtin_amdgcn_readfirstlane(n);
+    const long long br = __double_as_longlong(re), bi = __double_as_longlong(im);
+    re = __longlong_as_double(((long long)__builtin_amdgcn_readfirstlane((int)(br >> 32)) << 32)
+        | (unsigned int)__builtin_amdgcn_readfirstlane((int)(br & 0xffffffff)));
+    im = __longlong_as_double(((long long)__builtin_amdgcn_readfirstlane((int)(bi >> 32)) << 32)
+        | (unsigned int)__builtin_amdgcn_readfirstlane((int)(bi & 0xffffffff)));
+}
+
+// Complex sums over the first 16 lanes (row_sum2) or the whole wavefront
+// (wave_sum2), as row_sum3 / wave_sum3 without the count.
+__device__ inline double dpp_rfl(double v)
+{
+    const long long b = __double_as_longlong(v);
+    return __longlong_as_double(((long long)__builtin_amdgcn_readfirstlane((int)(b >> 32)) << 32)
+        | (unsigned int)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)));
+}
+__device__ inline void row_sum2(double& re, double& im)
+{
+#define SMIP_LEVEL(CTRL)                                                                                    \
+    {                                                                                                       \
+        const double tr = dpp_row_d<CTRL>(re), ti = dpp_row_d<CTRL>(im);                                    \
+        re += tr;                                                                                           \
+        im += ti;                                                                                           \
+    }
+    SMIP_LEVEL(0xB1)  // quad_perm [1,0,3,2]
+    SMIP_LEVEL(0x4E)  // quad_perm [2,3,0,1]
+    SMIP_LEVEL(0x141) // row_half_mirror
+    SMIP_LEVEL(0x140) // row_mirror
+#undef SMIP_LEVEL
+    re = dpp_rfl(re); // lane 0 holds the row's total
+    im = dpp_rfl(im);
+}
+__device__ inline void wave_sum2(double& re, double& im)
+{
+    int n = 0;
+    wave_sum3(n, re, im);
+}
+
+// Cross-wave signalling through LDS.  A wavefront's LDS instructions execute
+// in issue order, so a flag written after its data and read before it orders
+// them.  The flags are relaxed atomics (read afresh, never cached in a
+// register) and signal fences keep the compiler from moving other accesses
+// across them; unlike volatile accesses, neither adds a wait for the access to
+// complete, which would cost the chain an LDS round trip each time.
+__device__ inline int32_t lds_get(const int32_t* p)
+{
+    return __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ inline void lds_put(int32_t* p, int32_t v)
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// the ready word of a visit's helper header
+__device__ inline int32_t* ready_flag(i32x4* head, int k) { return reinterpret_cast<int32_t*>(&head[k]) + 2; }
+
+// Waits until *v >= want.  False when the workgroup aborted meanwhile, or when
+// the poll count ran out: then the abort flag and status[0] (if still clear) are
+// set, so every wave leaves its loop and the kernel ends.
+__device__ inline bool wait_at_least(const int32_t* v, int32_t want, int32_t* abort_flag, int32_t* status, int lane)
+{
+    for (uint32_t it = 0;; ++it) {
+        if (lds_get(v) >= want) {
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            return true;
+        }
+        if (lds_get(abort_flag))
+            return false;
+        if (it >= kSpinLimit) {
+            if (lane == 0)
+                atomicCAS(status, 0, SMIP_ERR_RUNTIME);
+            lds_put(abort_flag, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
+// One side of a pending record, as a helper sees it for visit k from the
+// snapshot after visit p - 1 (p = k - kLead): 0 flagged before p (it stays
+// flagged), 1 undecided (the point is visited in [q, k), q = max(p, 0), so it may
+// be flagged by visit k), 2 not flagged at visit k.
+__device__ inline int side_state(bool pre, int32_t j, int32_t p, int32_t q, const int32_t* ftime,
+    const int32_t (&fpw)[kLead])
+{
+    if (pre)
+        return 0;
+    j = max(j, 0);
+    if (ftime[j] < p)
+        return 0;
+    bool seen = j >= q; // first visit of the point inside [q, k)
+#pragma unroll
+    for (int i = 0; i < kLead; ++i)
+        seen |= fpw[i] == j; // a later visit of it inside [q, k)
+    return seen ? 1 : 2;
+}
+
+// The recursion over one window, by one workgroup of kFixWaves wavefronts.
+//
+// Wave 0 (the chain) walks the visits in order.  For visit k it needs the sum
+// over the visit's pending records whose points are flagged by now.  Helper
+// waves take visits k = h, h + kHelpers, ... ahead of the chain: once the chain
+// has finished visit p - 1 (p = k - kLead), a record both of whose in-window
+// points were flagged before p is certain to count, and one with a point neither
+// flagged before p nor visited in [p, k) is certain not to; the helper sums the
+// first kind into far[k] and copies only the undecided rest ("near" records) to
+// LDS.  The chain then adds the few near records whose points are flagged at
+// visit k and finalises the point as calc_phase does.  Which record lands where
+// depends only on the visit list and on when each slot was flagged (ftime), so
+// the summation order, and every bit of the result, is the same on every run.
 __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
     double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
-    __shared__ PendingRec buf[2][kChunkRecs]; // double-buffered record staging
     __shared__ StepInfo info[kWindow];
-    __shared__ int32_t ctotal[kWindow / kChunk];
-    // phase and consistency of each window slot once flagged in this window;
-    // ppos x = kNotYet before (no unit or zero phase has x = 2).  The window's
-    // results go to global memory once, at the end: within the window every
-    // reader of them goes through these arrays.
+    // phase, consistency and flagging visit of each window slot once flagged in
+    // this window (ppos x = kNotYet before: no unit or zero phase has x = 2).
+    // The window's results go to global memory once, at the end.
     __shared__ double2 ppos[kWindow];
     __shared__ double pcons[kWindow];
-    const int lane = threadIdx.x;
-    if (status[0])
-        return;
+    __shared__ int32_t ftime[kWindow];
+    __shared__ double2 far_s[kWindow];
+    // per visit: far count + (errors << 16), near count (-1: the chain takes
+    // every record of the visit), ready flag; written as one 16-byte store
+    // after far_s, read as one 16-byte load before it
+    __shared__ i32x4 head[kWindow];
+    __shared__ PendingRec nearbuf[kRing][kNearCap];
+    __shared__ int32_t progress, abort_flag;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nsteps = (int)(e - s);
-    const int nchunks = (nsteps + kChunk - 1) / kChunk;
-    // per-visit scalars of the whole window: independent loads, all in flight at once
-    {
-        StepInfo sv[kWindow / 64];
-#pragma unroll
-        for (int q = 0; q < kWindow / 64; ++q) {
-            const int t = min(lane + 64 * q, nsteps - 1); // nsteps >= 1; a clamped lane stores nothing
-            StepInfo& si = sv[q];
-            si.n0 = N[t];
-            si.fp = first_pos[s + t];
-            si.cnt = cnt[t];
-            si.err = E[t];
-            si.rev = si.fp != t;
-            const int2 w = visits[s + t];
-            si.pw = pm_at(a, w.x, w.y);
-            si.S = S[t];
-            si.off = 0;
-            si.pad = 0;
-        }
-#pragma unroll
-        for (int q = 0; q < kWindow / 64; ++q)
-            if (lane + 64 * q < nsteps)
-                info[lane + 64 * q] = sv[q];
+    if (tid == 0) {
+        progress = 0;
+        abort_flag = status[0] != 0; // an earlier window raised
     }
-    for (int t = lane; t < kWindow; t += 64)
+    for (int t = tid; t < nsteps; t += kFixThreads) {
+        StepInfo si;
+        si.n0 = N[t];
+        si.fp = first_pos[s + t];
+        si.cnt = cnt[t];
+        si.err = E[t];
+        si.S = S[t];
+        info[t] = si;
+    }
+    for (int t = tid; t < kWindow; t += kFixThreads) {
         ppos[t] = mk<double2>(kNotYet, 0.0);
-    __syncthreads();
-    for (int c = lane; c < nchunks; c += 64) { // chunk-local record offsets
-        int o = 0;
-        for (int q = c * kChunk; q < min(nsteps, (c + 1) * kChunk); ++q) {
-            info[q].off = o;
-            o += info[q].cnt;
-        }
-        ctotal[c] = o;
+        ftime[t] = kNever;
+        head[t] = i32x4 { 0, 0, 0, 0 };
     }
     __syncthreads();
 
-    const int4* src4 = reinterpret_cast<const int4*>(rec);
-    if (nchunks > 0 && ctotal[0] <= kChunkRecs)
-        fix_stage_chunk(reinterpret_cast<int4*>(buf[0]), src4, info, 0, nsteps, ctotal[0] * kParts, cap, lane);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
     int calls = 0, flagged = 0;
-    bool aborted = false;
-    for (int c = 0; c < nchunks && !aborted; ++c) {
-        const PendingRec* cb = buf[c & 1];
-        const bool staged = ctotal[c] <= kChunkRecs;
-        // the next chunk's records travel while this one is processed
-        if (c + 1 < nchunks && ctotal[c + 1] <= kChunkRecs)
-            fix_stage_chunk(reinterpret_cast<int4*>(buf[(c + 1) & 1]), src4, info, c + 1, nsteps,
-                ctotal[c + 1] * kParts, cap, lane);
-        const int q0 = c * kChunk, q1 = min(nsteps, q0 + kChunk);
-        // The chunk's step scalars, read once: the visit loop below is unrolled, so
-        // each visit's stay in registers of their own, and a visit's only LDS round
-        // trip is the one that must follow its predecessor's write (partner phases,
-        // the revisit check, and the next visit's records behind them).
-        int32_t n0[kChunk], fp[kChunk], cn[kChunk], of[kChunk], er[kChunk], rv[kChunk];
-        double2 Sv[kChunk];
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const bool in = q0 + k < q1;
-            const StepInfo si = info[in ? q0 + k : q0];
-            n0[k] = in ? __builtin_amdgcn_readfirstlane(si.n0) : -1;
-            fp[k] = __builtin_amdgcn_readfirstlane(si.fp);
-            cn[k] = in ? __builtin_amdgcn_readfirstlane(si.cnt) : 0;
-            of[k] = __builtin_amdgcn_readfirstlane(si.off);
-            er[k] = __builtin_amdgcn_readfirstlane(si.err);
-            rv[k] = __builtin_amdgcn_readfirstlane(si.rev);
-            Sv[k] = si.S;
-        }
-        PendingRec nr {};
-        if (staged && lane < cn[0])
-            nr = cb[of[0] + lane];
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const int j = q0 + k;
-            const PendingRec r0 = nr; // this visit's first 64 records, prefetched from LDS
-            double2 pu0 = mk<double2>(kNotYet, 0.0), pv0 = mk<double2>(kNotYet, 0.0);
-            if (staged && lane < cn[k]) {
-                pu0 = ppos[max(r0.ju, 0)];
-                pv0 = ppos[max(r0.jv, 0)];
+    if (!lds_get(&abort_flag)) {
+        auto take = [&](const PendingRec& r, double2 pu, double2 pv, double& sr, double& sim, int& ne) {
+            const bool uok = (r.f & kPreU) || (r.ju >= 0 && pu.x != kNotYet);
+            const bool vok = (r.f & kPreV) || (r.jv >= 0 && pv.x != kNotYet);
+            if (!(uok && vok))
+                return;
+            if (r.f & kErr) {
+                ne += 1 << 16;
+                return;
             }
-            const double fpx = ppos[fp[k]].x; // flagged by an earlier visit of this window?
-            if (k + 1 < kChunk) {
-                nr = PendingRec {};
-                if (staged && lane < cn[k + 1])
-                    nr = cb[of[k + 1] + lane];
+            if (r.f & kValid) {
+                double2 qv = r.c;
+                if (!(r.f & kPreU))
+                    qv = cmul(pu, qv);
+                if (!(r.f & kPreV))
+                    qv = cmul(qv, pv);
+                qv = normalised(qv);
+                sr += qv.x;
+                sim += qv.y;
+                ++ne;
             }
-            if (n0[k] < 0)
-                continue;
-            if (rv[k] && fpx != kNotYet)
-                continue;
-            ++calls;
-            double sr = 0.0, sim = 0.0;
-            int ne = 0; // count + (error << 16)
-            auto take = [&](const PendingRec& r, double2 pu, double2 pv) {
-                const bool uok = (r.f & kPreU) || (r.ju >= 0 && pu.x != kNotYet);
-                const bool vok = (r.f & kPreV) || (r.jv >= 0 && pv.x != kNotYet);
-                if (!(uok && vok))
-                    return;
-                if (r.f & kErr) {
-                    ne += 1 << 16;
-                    return;
-                }
-                if (r.f & kValid) {
-                    double2 qv = r.c;
-                    if (!(r.f & kPreU))
-                        qv = cmul(pu, qv);
-                    if (!(r.f & kPreV))
-                        qv = cmul(qv, pv);
-                    qv = normalised(qv);
-                    sr += qv.x;
-                    sim += qv.y;
-                    ++ne;
-                }
+        };
+        if (wave == 0) {
+            // The chain, software-pipelined: while visit k is summed and
+            // finalised, the loads of visit k + 1 (its step scalars, the helper's
+            // results, its near records and the phases of their points as of
+            // visit k - 1) are in flight; visit k's own result is patched in
+            // from registers, so nothing on the step-to-step path waits for LDS.
+            // A single wavefront pays the issue latency of every instruction on
+            // this path, so the step is kept short: scalar loads for the step's
+            // scalars, one 16-byte load for the helper's counts, selects rather
+            // than branches.
+            __builtin_amdgcn_s_setprio(3);
+            struct Ahead {
+                StepInfo si;              // step scalars (the same in every lane)
+                i32x3 hd;                 // helper: far count, near count, ready
+                double2 fs, pfp;          // helper sum; phase of the visit's own slot
+                PendingRec r;             // this lane's near record (lane < near count)
+                double2 pu, pv;           // phases of its points
             };
-            auto take_at = [&](const PendingRec& r) { take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)]); };
-            if (staged) {
-                if (lane < cn[k])
-                    take(r0, pu0, pv0);
-                for (int t = 64 + lane; t < cn[k]; t += 64) // rare: more than 64 records
-                    take_at(cb[of[k] + t]);
-            } else {
-                for (int t = lane; t < cn[k]; t += 64)
-                    take_at(rec[(int64_t)j * cap + t]);
-            }
-            wave_sum3(ne, sr, sim);
-            if ((ne >> 16) || er[k]) {
-                if (lane == 0)
-                    status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
-                aborted = true;
-                break;
-            }
-            const int tn = n0[k] + (ne & 0xffff);
-            if (tn > 0) {
-                if (lane == 0) {
-                    double ap;
-                    ppos[fp[k]] = point_phase(Sv[k].x + sr, Sv[k].y + sim, tn, &ap);
-                    pcons[fp[k]] = ap;
+            // step scalars and helper results, then (in issue order behind the
+            // ready flag) the helper's data
+            auto load1 = [&](int k, Ahead& A) {
+                A.si = info[k];
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                // one 12-byte load: the ready word with the counts (the fourth word
+                // is not loaded, so no register of the load is reused before it lands)
+                A.hd = *reinterpret_cast<const i32x3*>(&head[k]);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                A.fs = far_s[k];
+                A.r = nearbuf[k % kRing][lane & (kNearCap - 1)];
+            };
+            // phases as written so far (the caller patches in later visits)
+            auto load2 = [&](Ahead& A) {
+                A.pfp = ppos[A.si.fp];
+                A.pu = ppos[max(A.r.ju, 0)];
+                A.pv = ppos[max(A.r.jv, 0)];
+            };
+            double2 ph_prev = mk<double2>(kNotYet, 0.0);
+            int fp_prev = -1;
+            bool stop = false;
+            auto visit = [&](int k, Ahead& A, Ahead& B) {
+                if (fp_prev >= 0) { // the previous visit flagged slot fp_prev
+                    if (A.si.fp == fp_prev)
+                        A.pfp = ph_prev;
+                    if (A.r.ju == fp_prev)
+                        A.pu = ph_prev;
+                    if (A.r.jv == fp_prev)
+                        A.pv = ph_prev;
                 }
-                ++flagged;
-                wave_lds_order();
+                const int n0 = __builtin_amdgcn_readfirstlane(A.si.n0), fp = __builtin_amdgcn_readfirstlane(A.si.fp);
+                const bool run = n0 >= 0 && !(fp != k && __builtin_amdgcn_readfirstlane(A.pfp.x != kNotYet));
+                if (__builtin_expect(run && !__builtin_amdgcn_readfirstlane(A.hd.z), 0)) {
+                    // the helper was not done when the loads left: wait, then load
+                    // again (every earlier visit's phase is in LDS by now)
+                    if (!wait_at_least(ready_flag(head, k), 1, &abort_flag, status, lane)) {
+                        stop = true;
+                        return;
+                    }
+                    load1(k, A);
+                    load2(A);
+                }
+                const bool more = k + 1 < nsteps;
+                if (more)
+                    load1(k + 1, B);
+                fp_prev = -1;
+                if (run) {
+                    ++calls;
+                    const int nn = __builtin_amdgcn_readfirstlane(A.hd.y);
+                    double sr = 0.0, sim = 0.0;
+                    int nval = 0, nerr = 0;
+                    if (__builtin_expect(nn >= 0, 1)) {
+                        // near records, one per lane, as take() does, with selects
+                        const PendingRec& r = A.r;
+                        const bool pre_u = r.f & kPreU, pre_v = r.f & kPreV;
+                        const bool uok = pre_u || (r.ju >= 0 && A.pu.x != kNotYet);
+                        const bool vok = pre_v || (r.jv >= 0 && A.pv.x != kNotYet);
+                        const bool ok = lane < nn && uok && vok;
+                        const bool is_err = ok && (r.f & kErr);
+                        const bool is_val = ok && !(r.f & kErr) && (r.f & kValid);
+                        const double2 qa = pre_u ? mk<double2>(1.0, 0.0) : A.pu;
+                        const double2 qb = pre_v ? mk<double2>(1.0, 0.0) : A.pv;
+                        const double2 q = cmul(cmul(qa, r.c), qb);
+                        const double m2 = q.x * q.x + q.y * q.y;
+                        double inv = 1.5 - 0.5 * m2; // normalised()'s one Newton step from 1
+                        const bool slow = is_val && !(fabs(m2 - 1.0) < 1e-8);
+                        if (__builtin_expect(__ballot(slow) != 0, 0)) {
+                            if (slow)
+                                inv = 1.0 / sqrt(m2);
+                        }
+                        sr = is_val ? q.x * inv : 0.0;
+                        sim = is_val ? q.y * inv : 0.0;
+                        nval = __popcll(__ballot(is_val));
+                        nerr = __popcll(__ballot(is_err));
+                        if (__builtin_expect(nn > 16, 0))
+                            wave_sum2(sr, sim);
+                        else
+                            row_sum2(sr, sim);
+                    } else {
+                        // the helper left every record of this visit to the chain
+                        int ne = 0;
+                        const int cn = __builtin_amdgcn_readfirstlane(A.si.cnt);
+                        for (int t = lane; t < cn; t += 64) {
+                            const PendingRec r = rec[(int64_t)k * cap + t];
+                            take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)], sr, sim, ne);
+                        }
+                        wave_sum3(ne, sr, sim);
+                        nval = ne & 0xffff;
+                        nerr = ne >> 16;
+                    }
+                    const int fn = __builtin_amdgcn_readfirstlane(A.hd.x);
+                    if (__builtin_expect((fn >> 16) || nerr || __builtin_amdgcn_readfirstlane(A.si.err), 0)) {
+                        if (lane == 0)
+                            status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
+                        lds_put(&abort_flag, 1);
+                        stop = true;
+                        return;
+                    }
+                    if (more)
+                        load2(B); // phases as of visit k - 1
+                    const int tn = n0 + (fn & 0xffff) + nval;
+                    if (tn > 0) {
+                        double ap;
+                        const double2 ph = point_phase((A.si.S.x + A.fs.x) + sr, (A.si.S.y + A.fs.y) + sim, tn, &ap);
+                        if (lane == 0) {
+                            ppos[fp] = ph;
+                            pcons[fp] = ap;
+                            ftime[fp] = k;
+                        }
+                        ph_prev = ph;
+                        fp_prev = fp;
+                        ++flagged;
+                    }
+                } else if (more) {
+                    load2(B);
+                }
+                if (lane == 0)
+                    lds_put(&progress, k + 1);
+            };
+            // two visits per iteration, so the two prefetch sets swap roles
+            // instead of being copied
+            Ahead A, B;
+            if (nsteps > 0) {
+                load1(0, A);
+                load2(A);
+            }
+            for (int k = 0; k < nsteps && !stop; k += 2) {
+                visit(k, A, B);
+                if (!stop && k + 1 < nsteps)
+                    visit(k + 1, B, A);
+            }
+        } else {
+            // a helper: visits k = wave - 1 + i * kHelpers
+            for (int k = wave - 1; k < nsteps; k += kHelpers) {
+                const int n0 = __builtin_amdgcn_readfirstlane(info[k].n0);
+                if (n0 < 0) { // the chain does not look at it
+                    if (lane == 0)
+                        lds_put(ready_flag(head, k), 1);
+                    continue;
+                }
+                const int cn = __builtin_amdgcn_readfirstlane(info[k].cnt);
+                const PendingRec* rk = rec + (int64_t)k * cap;
+                // the first 128 records travel while the helper waits
+                PendingRec pre[2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    pre[c] = PendingRec {};
+                    if (64 * c + lane < cn)
+                        pre[c] = rk[64 * c + lane];
+                }
+                const int p = k - kLead, q = max(p, 0);
+                if (p > 0 && !wait_at_least(&progress, p, &abort_flag, status, lane))
+                    break;
+                // the ring slot's previous visit is published (the chain may have
+                // skipped that visit without waiting for it)
+                if (k >= kRing && !wait_at_least(ready_flag(head, k - kRing), 1, &abort_flag, status, lane))
+                    break;
+                int32_t fpw[kLead];
+#pragma unroll
+                for (int i = 0; i < kLead; ++i)
+                    fpw[i] = q + i < k ? __builtin_amdgcn_readfirstlane(info[q + i].fp) : -2;
+                double sr = 0.0, sim = 0.0;
+                int ne = 0, nnear = 0;
+                PendingRec* nb = nearbuf[k % kRing];
+                for (int t0 = 0; t0 < cn; t0 += 64) {
+                    const int t = t0 + lane;
+                    PendingRec r;
+                    if (t0 < 128)
+                        r = pre[t0 >> 6];
+                    else
+                        r = t < cn ? rk[t] : PendingRec {};
+                    int cls = 2;
+                    if (t < cn)
+                        cls = max(side_state(r.f & kPreU, r.ju, p, q, ftime, fpw),
+                            side_state(r.f & kPreV, r.jv, p, q, ftime, fpw));
+                    if (cls == 0)
+                        take(r, ppos[max(r.ju, 0)], ppos[max(r.jv, 0)], sr, sim, ne);
+                    const uint64_t m = __ballot(cls == 1);
+                    const int idx = nnear + __popcll(m & ((1ull << lane) - 1));
+                    if (cls == 1 && idx < kNearCap)
+                        nb[idx] = r;
+                    nnear += __popcll(m);
+                }
+                wave_sum3(ne, sr, sim);
+                if (nnear > kNearCap) { // the chain takes every record of this visit
+                    sr = sim = 0.0;
+                    ne = 0;
+                    nnear = -1;
+                }
+                if (lane == 0) {
+                    far_s[k] = mk<double2>(sr, sim);
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    head[k] = i32x4 { ne, nnear, 1, 0 }; // one 16-byte store, behind far_s
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                }
             }
         }
-        __builtin_amdgcn_s_waitcnt(0); // the next chunk's records have landed
-        __syncthreads();
     }
+    __syncthreads();
+    const bool aborted = lds_get(&abort_flag) != 0;
     // the window's results, once
-    for (int t = lane; t < nsteps; t += 64) {
+    for (int t = tid; t < nsteps; t += kFixThreads) {
         const double2 ph = ppos[t];
         if (ph.x != kNotYet) {
-            const int32_t pw = info[t].pw;
+            const int2 w = visits[s + t];
+            const int32_t pw = pm_at(a, w.x, w.y);
             flags[pw] = 1;
             cons[pw] = pcons[t];
             phases[pw] = ph;
@@ -645,13 +880,13 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
     // bulk pass: nothing in this window reads the map any more
     if (!aborted) {
         const int64_t s1 = e, e1 = min(e + (int64_t)kWindow, a.nvisits);
-        for (int64_t k = s1 + lane; k < e1; k += 64)
+        for (int64_t k = s1 + tid; k < e1; k += kFixThreads)
             if (first_pos[k] == (int32_t)(k - s1)) {
                 const int2 w = visits[k];
                 map[pm_at(a, w.x, w.y)] = ((win + 1) << 32) | (k - s1);
             }
     }
-    if (lane == 0) {
+    if (tid == 0) {
         atomicAdd(&status[1], calls);
         atomicAdd(&status[2], flagged);
     }
@@ -672,7 +907,7 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const in
     reco_bulk_body(blockIdx.x, a, visits, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec, cap, status);
 }
 
-__global__ __launch_bounds__(64) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
+__global__ __launch_bounds__(kFixThreads) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
@@ -724,7 +959,7 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* 
         global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
 }
 
-__global__ __launch_bounds__(64) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
+__global__ __launch_bounds__(kFixThreads) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
 {
     const RecoJob& J = jobs[blockIdx.x];
     const RecoArgs a = J.a;
@@ -1059,7 +1294,7 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
                     s0, e0, win, w->map);
             hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, s0, win, w->map, flags,
                 phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
-            hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(64), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
+            hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(kFixThreads), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
                 w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status, win, w->map);
         }
         SMIP_HIP(hipGetLastError());
@@ -1068,8 +1303,15 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
     SMIP_HIP(hipMemcpyAsync(st, w->status, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
     SMIP_HIP(hipStreamSynchronize(s));
     if (st[0])
-        fail(st[0], "reconstruct_phases: bispectrum element address out of bounds");
+        fail(st[0], reco_error_text(st[0]));
     return st[1];
+}
+
+const char* reco_error_text(int32_t code)
+{
+    if (code == SMIP_ERR_OUT_OF_RANGE)
+        return "reconstruct_phases: bispectrum element address out of bounds";
+    return "reconstruct_phases: the fixup's waves stopped waiting for each other (internal error)";
 }
 
 // Phase recovery of n bispectra at once: per window index one map, one bulk and
@@ -1118,7 +1360,7 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
             if (win == 0) // later windows' maps are written by the fixups before them
                 hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
             hipLaunchKernelGGL(k_reco_bulk_many, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
-            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(64), 0, s, dtab, win);
+            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(kFixThreads), 0, s, dtab, win);
         }
         SMIP_HIP(hipGetLastError());
         std::vector<int32_t> st(4 * (size_t)n);

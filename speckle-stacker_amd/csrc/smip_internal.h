@@ -418,6 +418,8 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
     hipStream_t s);
 void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s);
+// message for a nonzero recovery status code
+const char* reco_error_text(int32_t code);
 int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uint8_t* flags, double* consistency,
     hipStream_t s);
 

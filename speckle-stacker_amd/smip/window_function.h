@@ -1,7 +1,8 @@
 // window_function.h -- apodisation windows (reference interface:
 // include/window_function.h:11-84).  Values from libsmip_mi355x.so's host
-// routine: w(i, j) = (a + (1-a) cos(2 pi i / A)) (a + (1-a) cos(2 pi j / A)) for
-// |i|, |j| <= A/2 and 0 outside, signed FFT-order indices.
+// routine: w(i, j) = (a + (1-a) cos(2 pi i / A)) (a + (1-a) cos(2 pi j / A)) at
+// every signed FFT-order index (i, j) of the array, with no cut-off at |i| = A/2:
+// beyond it the cosine is evaluated as it is inside, as the reference does.
 #pragma once
 
 #include <stdexcept>
