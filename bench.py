@@ -643,6 +643,37 @@ def batch_leg_grouped(ctx, nseq: int, batches_per_call: int = 1, window: int = 0
                     "thread; grouped phase recovery"}
 
 
+def fft_passes_leg(ctx) -> dict:
+    """The transform passes alone: b = 2 leaves the triple products negligible,
+    so a batch's accumulate time is its forward FFT and |F|^2, and with
+    shift-and-add on its SAA time is the FP64 FFT, correlation, inverse with
+    argmax and the shifted sum.  512 frames per batch of the 256x256 uint16 and
+    512x512 float32 stacks."""
+    out = {}
+    for name, N, passes in (("triple256", 256, 4), ("extended512", 512, 2)):
+        fr = S.synth_frames(S.synth_params(name), 0, 512, threads=min(16, os.cpu_count() or 1))
+        dev = S.Frames(ctx, fr)
+        for saa in (False, True):
+            job = S.Job(ctx, S.job_params(N, N, 2, 2, fr.dtype, S.PRECISION_MIXED, saa=saa, frames_per_batch=512))
+            job.add_frames(dev)
+            ctx.synchronize()
+            st0 = job.stats()
+            t0 = time.perf_counter()
+            for _ in range(passes):
+                job.add_frames(dev)
+            ctx.synchronize()
+            wall = time.perf_counter() - t0
+            st = job.stats()
+            nf = 512 * passes
+            out[f"{name}{'_saa' if saa else ''}"] = {
+                "frames": nf, "wall_us_per_frame": round(wall / nf * 1e6, 3),
+                "accumulate_us_per_frame": round((st["accumulate_ms"] - st0["accumulate_ms"]) / nf * 1e3, 3),
+                "saa_us_per_frame": round((st["saa_ms"] - st0["saa_ms"]) / nf * 1e3, 3)}
+            job.close()
+        dev.close()
+    return out
+
+
 def phase_bins(m, d, u, fl) -> dict:
     """Median |phase difference| of the mixed path and of the one-ulp FP64 run
     against the FP64 path, over recovered points grouped by their FP64
@@ -774,7 +805,7 @@ def main() -> int:
     ap.add_argument("--c3-frames", type=int, default=512, help="frames of the 512x512 throughput sample")
     ap.add_argument("--no-configs", action="store_true", help="skip the 128x128 and 512x512 legs")
     ap.add_argument("--only", default="",
-                    help="comma list of legs to run: latency,configs,batch,precision,serve,cpu")
+                    help="comma list of legs to run: latency,configs,batch,precision,serve,cpu,fft")
     ap.add_argument("--precision-frames", type=int, default=10000,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
@@ -792,6 +823,9 @@ def main() -> int:
         args.batch_seqs = args.batch_seqs if "batch" in legs else 0
         args.precision_frames = args.precision_frames if "precision" in legs else 0
         args.serve_seconds = args.serve_seconds if "serve" in legs else 0
+        args.fft_passes = "fft" in legs
+    else:
+        args.fft_passes = False
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -960,15 +994,20 @@ def main() -> int:
         out["extended512"] = workload_leg(ctx, "extended512", args.c3_frames, passes=2, latency_runs=1)
         log(f"extended512: {out['extended512']}")
 
+    if args.fft_passes:
+        log("the transform passes alone (b = 2)")
+        out["fft_passes"] = fft_passes_leg(ctx)
+        log(f"fft passes: {out['fft_passes']}")
+
     if args.batch_seqs > 0:
         log(f"multi-sequence: {args.batch_seqs} sequences, 16 host threads")
         out["multi_sequence"] = batch_leg(ctx, args.batch_seqs, 16, plain_sample=32)
         log(f"multi-sequence: {out['multi_sequence']}")
         log(f"multi-sequence, grouped launches: {args.batch_seqs} sequences")
-        for bpc in (2, 4):
+        for bpc in (1, 4):
             gg = batch_leg_grouped(ctx, args.batch_seqs, bpc)
             log(f"multi-sequence, grouped launches, {bpc} batches per call: {gg}")
-        g = batch_leg_grouped(ctx, args.batch_seqs)
+        g = batch_leg_grouped(ctx, args.batch_seqs, 2)
         g["speedup_vs_plain_fp64_serial"] = round(out["multi_sequence"]["plain_fp64_serial_s_estimate"]
                                                   / g["makespan_s"], 2)
         out["multi_sequence_grouped"] = g

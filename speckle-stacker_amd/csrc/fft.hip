@@ -177,9 +177,26 @@ __device__ V* lds_fft(V* a, V* b, int n, int log2n, int nb, const V* __restrict_
     return a;
 }
 
+// Column tiles in LDS.  Element y of column c sits at c * col_ld<V>(Y) +
+// col_at<V>(y): one padding element after every 2^col_shift elements (every 64
+// bytes of float2, every 256 bytes of double2) and one more per column.  Without
+// padding, a stride of Y elements is a multiple of the 64 banks' 256 bytes: the
+// transposing stores and loads of a wavefront that walks the columns all hit one
+// bank, and the late radix-4 stages, whose lanes step over runs of Q < 16
+// elements, hit a few; with it, a model of the 64 banks puts every stage of the
+// 256- and 512-point double2 transforms at the conflict-free minimum and the
+// float2 ones within 12% of it.
+template <typename V>
+__host__ __device__ constexpr int col_shift() { return sizeof(V) == 16 ? 4 : 3; }
+template <typename V>
+__host__ __device__ constexpr int col_ld(int Y) { return Y + (Y >> col_shift<V>()) + 1; }
+template <typename V>
+__device__ inline int col_at(int i) { return i + (i >> col_shift<V>()); }
+
 // ---------------------------------------------------------------------------
 // In-place LDS FFT (decimation in frequency) of nb transforms of length
-// n = 2^log2n stored back to back in a: a radix-2 stage first when log2n is odd,
+// n = 2^log2n, element i of transform f at a[f * ld + col_at<V>(i)] (a column
+// tile as laid out above): a radix-2 stage first when log2n is odd,
 // then radix-4 stages.  Every butterfly reads and writes the same four (two)
 // locations, so one buffer and one barrier per stage suffice -- half the LDS of
 // the Stockham form, which lets a column pass hold twice the columns.  The
@@ -189,7 +206,7 @@ __device__ V* lds_fft(V* a, V* b, int n, int log2n, int nb, const V* __restrict_
 //   radix-2 stage: x_0 + x_1, (x_0 - x_1) w_n^j
 // ---------------------------------------------------------------------------
 template <typename V>
-__device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict__ tw, bool inv)
+__device__ void lds_fft_dif(V* a, int n, int log2n, int nb, int ld, const V* __restrict__ tw, bool inv)
 {
     const int nthreads = blockDim.x;
     const int tid = threadIdx.x;
@@ -198,13 +215,14 @@ __device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict_
         const int h = n >> 1;
         for (int t = tid; t < nb * h; t += nthreads) {
             const int f = t >> (log2n - 1), j = t & (h - 1);
-            V* x = a + f * n + j;
-            const V x0 = x[0], x1 = x[h];
+            V* x0p = a + f * ld + col_at<V>(j);
+            V* x1p = a + f * ld + col_at<V>(j + h);
+            const V x0 = *x0p, x1 = *x1p;
             V w = tw[j];
             if (inv)
                 w = cconj(w);
-            x[0] = cadd(x0, x1);
-            x[h] = cmul(csub(x0, x1), w);
+            *x0p = cadd(x0, x1);
+            *x1p = cmul(csub(x0, x1), w);
         }
         __syncthreads();
         L = h;
@@ -216,8 +234,13 @@ __device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict_
         for (int t = tid; t < nb * (n >> 2); t += nthreads) {
             const int f = t >> (log2n - 2), b = t & ((n >> 2) - 1);
             const int g = b >> lQ, j = b & (Q - 1);
-            V* x = a + f * n + g * L + j;
-            const V x0 = x[0], x1 = x[Q], x2 = x[2 * Q], x3 = x[3 * Q];
+            V* col = a + f * ld;
+            const int i0 = g * L + j;
+            V* p0 = col + col_at<V>(i0);
+            V* p1 = col + col_at<V>(i0 + Q);
+            V* p2 = col + col_at<V>(i0 + 2 * Q);
+            V* p3 = col + col_at<V>(i0 + 3 * Q);
+            const V x0 = *p0, x1 = *p1, x2 = *p2, x3 = *p3;
             const V a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
             const V a3 = inv ? cmul_pi(csub(x1, x3)) : cmul_mi(csub(x1, x3));
             V y1 = cadd(a1, a3), y2 = csub(a0, a2), y3 = csub(a1, a3);
@@ -232,10 +255,10 @@ __device__ void lds_fft_dif(V* a, int n, int log2n, int nb, const V* __restrict_
                 y2 = cmul(y2, w2);
                 y3 = cmul(y3, w3);
             }
-            x[0] = cadd(a0, a2);
-            x[Q] = y1;
-            x[2 * Q] = y2;
-            x[3 * Q] = y3;
+            *p0 = cadd(a0, a2);
+            *p1 = y1;
+            *p2 = y2;
+            *p3 = y3;
         }
         __syncthreads();
         L = Q;
@@ -267,9 +290,9 @@ __device__ inline S load_sample(const In* p, int64_t i, bool absval)
 
 // LDS budget per workgroup (two ping-pong buffers): 64 KiB.
 constexpr int kLdsBytes = 65536;
-// LDS of a column-pass workgroup.  Measured against 64 KiB (and against the
-// two-buffer Stockham form at 32 KiB): the shift-and-add at 512x512 is 8% faster,
-// at 128x128 13% slower, elsewhere within 2%.
+// Data bytes of a column-pass tile (the padding of col_ld comes on top).
+// Measured against 64 KiB: the shift-and-add at 512x512 and 256x256 is 9-10%
+// slower there (half the workgroups per CU), though its rows fill cache lines.
 constexpr int kColLds = 32768;
 constexpr int kThreads = 256;
 // elements per thread of a column tile (cb * Y * sizeof(V) <= kColLds) and of a
@@ -376,6 +399,7 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
     const V* __restrict__ tw, Out* __restrict__ out, HalfPlane hp, int bx, int64_t f, char* smem)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
+    const int ld = col_ld<V>(Y);
     V* A = reinterpret_cast<V*>(smem);
     const int kx0 = bx * cb;
     gather_tile<kColPer<V>, V>(
@@ -387,10 +411,10 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
         },
         [&](int t, const V& v) {
             const int y = t >> lcb, c = t & (cb - 1);
-            A[c * Y + y] = v;
+            A[c * ld + col_at<V>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<V>(A, Y, ly, cb, tw, false);
+    lds_fft_dif<V>(A, Y, ly, cb, ld, tw, false);
     const V* R = A; // digit-reversed: X[k] at dif_pos(k)
     if (MODE == 0) {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -398,7 +422,7 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
-            const V v = R[c * Y + dif_pos((Y / 2 - r) & (Y - 1), ly)];
+            const V v = R[c * ld + col_at<V>(dif_pos((Y / 2 - r) & (Y - 1), ly))];
             Out o;
             o.x = v.x;
             o.y = -v.y;
@@ -410,7 +434,7 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
             const int kx = kx0 + c;
             if (kx >= hx)
                 continue;
-            const V v = R[c * Y + dif_pos(ky, ly)];
+            const V v = R[c * ld + col_at<V>(dif_pos(ky, ly))];
             Out o;
             o.x = v.x;
             o.y = v.y;
@@ -477,6 +501,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
     double2* __restrict__ G)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
+    const int ld = col_ld<double2>(Y);
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     int bx;
@@ -499,15 +524,15 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
         },
         [&](int t, const double2& v) {
             const int y = t >> lcb, c = t & (cb - 1);
-            A[c * Y + y] = v;
+            A[c * ld + col_at<double2>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
         if (kx < hx)
-            G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
+            G[(f * Y + y) * hx + kx] = A[c * ld + col_at<double2>(dif_pos(y, ly))];
     }
 }
 
@@ -520,6 +545,7 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
     const double2* __restrict__ tw, double2* __restrict__ G, int bx, int64_t f, char* smem)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
+    const int ld = col_ld<double2>(Y);
     constexpr int kPer = kColPer<double2>;
     double2* A = reinterpret_cast<double2*>(smem);
     const int kx0 = bx * cb;
@@ -532,10 +558,10 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
         },
         [&](int t, const double2& v) {
             const int y = t >> lcb, c = t & (cb - 1);
-            A[c * Y + y] = v;
+            A[c * ld + col_at<double2>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, tw, false);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, false);
     // back to natural order through registers, multiplied on the way
     double2 v[kPer];
 #pragma unroll
@@ -548,7 +574,7 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
             if (kx < hx) {
                 const double2 a = Rref[(int64_t)ky * hx + kx];
                 // std::conj(a) * b with the (ac - bd, ad + bc) operand order
-                v[q] = cmul(cconj(a), A[c * Y + dif_pos(ky, ly)]);
+                v[q] = cmul(cconj(a), A[c * ld + col_at<double2>(dif_pos(ky, ly))]);
             }
         }
     }
@@ -557,15 +583,15 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
     for (int q = 0; q < kPer; ++q) {
         const int t = threadIdx.x + q * kThreads;
         if (t < cb * Y)
-            A[t] = v[q];
+            A[(t >> ly) * ld + col_at<double2>(t & (Y - 1))] = v[q];
     }
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, tw, true);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
         if (kx < hx)
-            G[(f * Y + y) * hx + kx] = A[c * Y + dif_pos(y, ly)];
+            G[(f * Y + y) * hx + kx] = A[c * ld + col_at<double2>(dif_pos(y, ly))];
     }
 }
 
@@ -914,6 +940,7 @@ __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ dat
     const double2* __restrict__ tw, int inv)
 {
     const int lcb = __builtin_ctz(cb); // cb is a power of two
+    const int ld = col_ld<double2>(Y);
     extern __shared__ char smem[];
     double2* A = reinterpret_cast<double2*>(smem);
     const int x0 = blockIdx.x * cb;
@@ -925,14 +952,14 @@ __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ dat
         },
         [&](int t, const double2& v) {
             const int y = t >> lcb, c = t & (cb - 1);
-            A[c * Y + y] = v;
+            A[c * ld + col_at<double2>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, tw, inv != 0);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, inv != 0);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         if (x0 + c < X)
-            data[(int64_t)y * X + x0 + c] = A[c * Y + dif_pos(y, ly)];
+            data[(int64_t)y * X + x0 + c] = A[c * ld + col_at<double2>(dif_pos(y, ly))];
     }
 }
 
@@ -1020,7 +1047,7 @@ static void launch_cols_fwd(const FftPlan* p, const V* G, int64_t nf, Out* out, 
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<V>(Y);
     dim3 grid((hx + cb - 1) / cb, (unsigned)nf);
-    const size_t lds = sizeof(V) * cb * Y;
+    const size_t lds = sizeof(V) * cb * col_ld<V>(Y);
     hipLaunchKernelGGL((k_cols_fwd<V, MODE, Out>), grid, dim3(kThreads), lds, s, G, X, Y, p->ly, hx, cb, tw_y<V>(p),
         out, hp);
     SMIP_HIP(hipGetLastError());
@@ -1083,7 +1110,7 @@ void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t n
     launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
     const int cb = cols_per_wg<double2>(Y); // cb * Y * 16 <= kColLds, as k_xcorr_cols's registers assume
     hipLaunchKernelGGL(k_xcorr_cols, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        sizeof(double2) * cb * Y, s, ref_spec, X, Y, p->ly, hx, cb, p->twy64, work);
+        sizeof(double2) * cb * col_ld<double2>(Y), s, ref_spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;
@@ -1124,9 +1151,9 @@ FftTiling fft_tiling(const FftPlan* p)
     t.cb64 = cols_per_wg<double2>(Y);
     t.nblk64 = (Y / 2) / t.ppw64;
     t.lds_rows32 = 2 * sizeof(float2) * t.ppw32 * X;
-    t.lds_cols32 = sizeof(float2) * t.cb32 * Y;
+    t.lds_cols32 = sizeof(float2) * t.cb32 * col_ld<float2>(Y);
     t.lds_rows64 = 2 * sizeof(double2) * t.ppw64 * X;
-    t.lds_cols64 = sizeof(double2) * t.cb64 * Y;
+    t.lds_cols64 = sizeof(double2) * t.cb64 * col_ld<double2>(Y);
     return t;
 }
 
@@ -1140,7 +1167,7 @@ void launch_xcorr_f64(const FftPlan* p, const double2* ref_spec, const double2* 
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<double2>(Y);
     hipLaunchKernelGGL(k_xcorr_cols_inv, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        sizeof(double2) * cb * Y, s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
+        sizeof(double2) * cb * col_ld<double2>(Y), s, ref_spec, spec, X, Y, p->ly, hx, cb, p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     hipLaunchKernelGGL(k_rows_c2r, dim3((Y / 2) / ppw, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
@@ -1161,7 +1188,7 @@ void launch_c2c_inverse_f64(const FftPlan* p, double2* data, double2* work, hipS
         rpw, p->twx64, 1);
     SMIP_HIP(hipGetLastError());
     const int cb = cols_per_wg<double2>(Y);
-    hipLaunchKernelGGL(k_cols_c2c, dim3((X + cb - 1) / cb), dim3(kThreads), sizeof(double2) * cb * Y, s, data, X,
+    hipLaunchKernelGGL(k_cols_c2c, dim3((X + cb - 1) / cb), dim3(kThreads), sizeof(double2) * cb * col_ld<double2>(Y), s, data, X,
         Y, p->ly, cb, p->twy64, 1);
     SMIP_HIP(hipGetLastError());
 }
