@@ -244,14 +244,17 @@ def run_latency_jobs(ctx, dev, W, njobs: int, batch: int, vy: int, grouped: bool
 
 
 def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, camera_hz: float = 1000.0,
-              max_active: int = 64, drain_s: float = 90.0) -> dict:
+              max_active: int = 64, drain_s: float = 90.0, pushers: int = 4, workers: int = 0) -> dict:
     """The request loop under Poisson arrivals: jobs drawn as the multi-sequence
     mix (frame side from {64, 128, 256, 512}, frame count log-uniform in
     [40, 5000], b uniform in [16, 64], p = 2b, object and sample type per job),
     each streaming its frames in chunks of `chunk` at `camera_hz` per job.
-    Frames cycle through a pool of pre-generated frames per (side, type).  The
-    run stops `drain_s` after the last arrival; jobs not finished by then are
-    reported as unfinished."""
+    Frames cycle through a pool of pre-generated frames per (side, type).
+    `pushers` client threads push the jobs' chunks (job i by thread i mod
+    pushers), as separate cameras would: one thread's pushes from pageable memory
+    reach about 8.5 GB/s, about 40 k frames/s of this mix.  The run stops
+    `drain_s` after the last arrival; jobs not finished by then are reported as
+    unfinished."""
     import heapq
 
     rng = np.random.default_rng(seed)
@@ -266,73 +269,108 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
         q = draw_sequences(rng, 1)[0]
         q["t"] = t
         jobs.append(q)
-    srv = S.Server(ctx, max_active=max_active)
-    pending = []  # (time, seq, job index, first frame)
+    srv = S.Server(ctx, max_active=max_active, workers=workers)
+    pending = [[] for _ in range(pushers)]  # per pusher: (time, seq, job index, first frame)
     for i, j in enumerate(jobs):
-        heapq.heappush(pending, (j["t"], i, i, 0))
+        heapq.heappush(pending[i % pushers], (j["t"], i, i, 0))
     closed = []
     results, failures = {}, {}
     stop = threading.Event()
 
     def reaper():
+        # releases every closed job as soon as it is done, in whatever order they
+        # finish: a job's slot and memory return only at release, so waiting on the
+        # oldest closed job while it waits for a slot held by finished ones would
+        # stall the server
         k = 0
+        outstanding = []
         while True:
-            if k >= len(closed):
+            while k < len(closed):
+                outstanding.append(closed[k])
+                k += 1
+            if not outstanding:
                 if stop.is_set():
                     return
                 time.sleep(0.005)
                 continue
-            i, jid = closed[k]
-            try:
-                while srv.wait(jid, 0.5) not in (S.JOB_DONE, S.JOB_FAILED):
-                    if stop.is_set():
-                        return
-                results[i] = srv.info(jid)
-            except Exception as e:  # reported, not hidden
-                failures[i] = repr(e)
-            try:
-                srv.release(jid)
-            except Exception:
-                pass
-            k += 1
+            still = []
+            for i, jid in outstanding:
+                try:
+                    st = srv.wait(jid, 0.0)
+                    if st not in (S.JOB_DONE, S.JOB_FAILED):
+                        still.append((i, jid))
+                        continue
+                    results[i] = srv.info(jid)
+                except Exception as e:  # reported, not hidden
+                    failures[i] = repr(e)
+                try:
+                    srv.release(jid)
+                except Exception:
+                    pass
+            outstanding = still
+            if stop.is_set():
+                return
+            time.sleep(0.002)
 
     th = threading.Thread(target=reaper)
     th.start()
     t0 = time.perf_counter()
-    last_log = t0
-    frames_pushed = 0
-    seq = len(jobs)
+    pushed = [0] * pushers
+    lag = [0.0] * pushers
     deadline = t0 + seconds + drain_s
-    while pending and time.perf_counter() < deadline:
-        when, _, i, first = heapq.heappop(pending)
-        delay = t0 + when - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        j = jobs[i]
-        pool = pools[(j["N"], j["kind"])]
-        if first == 0:
-            j["id"] = srv.submit(S.job_params(j["N"], j["N"], j["b"], j["p"], pool.dtype))
-        cnt = min(chunk, j["n"] - first)
-        pos = 0
-        while pos < cnt:  # contiguous pieces of the cycled pool, pushed without a host copy here
-            a = (j["off"] + first + pos) % len(pool)
-            c = min(cnt - pos, len(pool) - a)
-            srv.push(j["id"], pool[a:a + c])
-            pos += c
-        frames_pushed += cnt
-        if first + cnt >= j["n"]:
-            srv.close_job(j["id"])
-            closed.append((i, j["id"]))
-        else:
-            seq += 1
-            heapq.heappush(pending, (when + cnt / camera_hz, seq, i, first + cnt))
+
+    def pusher(q):
+        heap = pending[q]
+        seq = len(jobs)
+        try:
+            while heap and time.perf_counter() < deadline:
+                when, _, i, first = heapq.heappop(heap)
+                delay = t0 + when - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                lag[q] = max(lag[q], time.perf_counter() - t0 - when)
+                j = jobs[i]
+                pool = pools[(j["N"], j["kind"])]
+                if first == 0:
+                    j["id"] = srv.submit(S.job_params(j["N"], j["N"], j["b"], j["p"], pool.dtype))
+                cnt = min(chunk, j["n"] - first)
+                pos = 0
+                while pos < cnt:  # contiguous pieces of the cycled pool, pushed without a host copy here
+                    a = (j["off"] + first + pos) % len(pool)
+                    c = min(cnt - pos, len(pool) - a)
+                    srv.push(j["id"], pool[a:a + c])
+                    pos += c
+                pushed[q] += cnt
+                if first + cnt >= j["n"]:
+                    srv.close_job(j["id"])
+                    closed.append((i, j["id"]))
+                else:
+                    seq += 1
+                    heapq.heappush(heap, (when + cnt / camera_hz, seq, i, first + cnt))
+        except Exception as e:  # reported, not hidden
+            failures[-1 - q] = repr(e)
+
+    pts = [threading.Thread(target=pusher, args=(q,)) for q in range(pushers)]
+    for p_ in pts:
+        p_.start()
+    last_log = t0
+    served = None  # (frames processed, seconds) at the end of the arrival window
+    while any(p_.is_alive() for p_ in pts):
+        time.sleep(0.25)
         now = time.perf_counter()
-        if now - last_log > 5.0:
+        if served is None and now - t0 >= seconds:
+            served = (srv.stats()["frames"], now - t0)
+        if now - last_log >= 5.0:
             st = srv.stats()
             log(f"serving t={now - t0:.0f}s: submitted {st['submitted']}, done {st['completed']}, active "
-                f"{st['active']}, waiting {st['waiting']}, frames {st['frames']}/{frames_pushed}, "
-                f"lag {max(0.0, now - t0 - when):.2f}s")
+                f"{st['active']}, waiting {st['waiting']}, frames {st['frames']}/{sum(pushed)}, "
+                f"lag {max(lag):.2f}s")
             last_log = now
+    for p_ in pts:
+        p_.join()
+    frames_pushed = sum(pushed)
+    if served is None:
+        served = (srv.stats()["frames"], time.perf_counter() - t0)
     while time.perf_counter() < deadline and len(results) + len(failures) < len(closed):
         time.sleep(1.0)
         st = srv.stats()
@@ -348,19 +386,63 @@ def serve_leg(ctx, seconds: float, rate: float, seed: int, chunk: int = 100, cam
     queue = np.array([results[i]["queue_ms"] for i in done]) if done else np.zeros(1)
     pct = lambda a, q: round(float(np.percentile(a, q)), 2)
     frames_done = int(sum(jobs[i]["n"] for i in done))
+    push_end_lag = round(max(lag), 2)
     return {"arrival": f"Poisson, {rate} jobs/s for {seconds} s", "jobs": len(jobs), "completed": len(done),
             "window": f"{seconds} s of arrivals (the configured serving run is 600 s; the same rate, mix and "
                       "camera model over a shorter window)",
             "failed": len(failures), "unfinished": len(jobs) - len(done) - len(failures),
             "failures": list(failures.values())[:3], "frames_of_completed_jobs": frames_done,
             "frames_pushed": int(frames_pushed), "wall_s": round(wall, 2),
-            "frames_per_s": round(st["frames"] / wall, 1), "max_active": max_active,
+            "frames_per_s": round(st["frames"] / wall, 1),
+            "served_frames_per_s_in_window": round(served[0] / served[1], 1), "max_active": max_active,
+            "workers": workers or 8,
             "latency_ms": {"p50": pct(lat, 50), "p95": pct(lat, 95), "p99": pct(lat, 99)},
             "after_last_frame_ms": {"p50": pct(tail, 50), "p95": pct(tail, 95), "p99": pct(tail, 99)},
             "admission_wait_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)},
-            "camera": f"chunks of {chunk} frames at {camera_hz:.0f} frames/s per job",
+            "camera": f"chunks of {chunk} frames at {camera_hz:.0f} frames/s per job, {pushers} pushing threads",
+            "push_lag_s": push_end_lag,
             "mix": "side {64,128,256,512}, frames log-uniform 40-5000, b U[16,64], p=2b, "
                    "binary/u8, triple/u16, extended/f32"}
+
+
+def serve_sweep(ctx, rates: list, seconds: float, plain_fps: float | None) -> dict:
+    """Serving capacity: serve_leg at rising Poisson rates, each over `seconds` of
+    arrivals with up to 60 s to drain.  Below capacity the completed frames/s
+    follow the offered load and the after-last-frame latency stays flat; past it
+    the jobs queue, that latency grows with the run and jobs are left unfinished.
+    The knee is the highest rate whose jobs all finish with an after-last-frame p99
+    under 1 s while the pushing threads keep to the arrival schedule (lag < 1 s)."""
+    runs = []
+    for k, rate in enumerate(rates):
+        r = serve_leg(ctx, seconds, rate, seed=0x5EC1E100 + k, drain_s=60.0)
+        offered = r["frames_pushed"] / seconds
+        runs.append({"rate_jobs_per_s": rate, "jobs": r["jobs"], "completed": r["completed"],
+                     "unfinished": r["unfinished"], "failed": r["failed"], "push_lag_s": r["push_lag_s"],
+                     "offered_frames_per_s": round(offered, 1), "frames_per_s": r["frames_per_s"],
+                     "served_frames_per_s_in_window": r["served_frames_per_s_in_window"],
+                     "after_last_frame_ms": r["after_last_frame_ms"], "latency_ms": r["latency_ms"],
+                     "admission_wait_ms": r["admission_wait_ms"]})
+        log(f"serving sweep {rate} jobs/s: {runs[-1]}")
+    ok = [x for x in runs if x["unfinished"] == 0 and x["failed"] == 0 and x["after_last_frame_ms"]["p99"] < 1000.0
+          and x["push_lag_s"] < 1.0]
+    knee = max(ok, key=lambda x: x["rate_jobs_per_s"]) if ok else None
+    out = {"window_s_per_rate": seconds, "runs": runs,
+           "knee_rate_jobs_per_s": knee["rate_jobs_per_s"] if knee else None,
+           "capacity_frames_per_s": knee["offered_frames_per_s"] if knee else None,
+           "rule": "knee = highest rate with every job finished, after-last-frame p99 < 1 s and the pushing "
+                   "threads within 1 s of the arrival schedule",
+           # above the knee the server runs flat out: the frames it has processed by the end of the
+           # arrival window, per second, no longer follow the offered load
+           "saturated_frames_per_s": max(x["served_frames_per_s_in_window"] for x in runs) if runs else None}
+    if knee and plain_fps:
+        out["plain_fp64_serial_frames_per_s"] = round(plain_fps, 1)
+        out["capacity_vs_plain_fp64_serial"] = round(knee["offered_frames_per_s"] / plain_fps, 2)
+        out["saturated_vs_plain_fp64_serial"] = round(out["saturated_frames_per_s"] / plain_fps, 2)
+        out["plain_note"] = ("the plain FP64 path run one sequence at a time over the same mix (multi-sequence "
+                             "leg's device-time estimate); its images are the FP64 reference the mixed path "
+                             "is held to (image rel L2 within 1e-5, or the FP64 path's own one-ulp sensitivity "
+                             "where that is larger)")
+    return out
 
 
 def draw_sequences(rng, n: int) -> list:
@@ -799,13 +881,16 @@ def main() -> int:
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson job arrivals per second")
+    ap.add_argument("--sweep-rates", default="20,30,40,50,60,80",
+                    help="Poisson rates of the serving capacity sweep (--only sweep)")
+    ap.add_argument("--sweep-seconds", type=float, default=8.0, help="arrival window per rate of the sweep")
     ap.add_argument("--batch-seqs", type=int, default=64, help="sequences of the multi-sequence leg (0: skip)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues per process for HIP (GPU_MAX_HW_QUEUES, at most 32; 0: HIP's default)")
     ap.add_argument("--c3-frames", type=int, default=512, help="frames of the 512x512 throughput sample")
     ap.add_argument("--no-configs", action="store_true", help="skip the 128x128 and 512x512 legs")
     ap.add_argument("--only", default="",
-                    help="comma list of legs to run: latency,configs,batch,precision,serve,cpu,fft")
+                    help="comma list of legs to run: latency,configs,batch,precision,serve,cpu,fft,sweep")
     ap.add_argument("--precision-frames", type=int, default=10000,
                     help="frames of the 512^2 precision comparison (0: skip)")
     args = ap.parse_args()
@@ -824,8 +909,10 @@ def main() -> int:
         args.precision_frames = args.precision_frames if "precision" in legs else 0
         args.serve_seconds = args.serve_seconds if "serve" in legs else 0
         args.fft_passes = "fft" in legs
+        args.sweep = "sweep" in legs
     else:
         args.fft_passes = False
+        args.sweep = False
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -1024,6 +1111,15 @@ def main() -> int:
         sv = out["serving"]
         log(f"serving: {sv['completed']}/{sv['jobs']} jobs, {sv['frames_per_s']} frames/s, latency {sv['latency_ms']}, "
             f"after last frame {sv['after_last_frame_ms']}")
+
+    if args.sweep:
+        rates = [float(x) for x in args.sweep_rates.split(",") if x]
+        ms = out.get("multi_sequence")
+        plain_fps = ms["frames"] / ms["plain_fp64_serial_s_estimate"] if ms else None
+        log(f"serving capacity sweep: {rates} jobs/s, {args.sweep_seconds} s each")
+        out["serving_capacity"] = serve_sweep(ctx, rates, args.sweep_seconds, plain_fps)
+        log(f"serving capacity: knee {out['serving_capacity']['knee_rate_jobs_per_s']} jobs/s, "
+            f"{out['serving_capacity']['capacity_frames_per_s']} frames/s")
 
     if not args.no_cpu:
         log("timing the CPU baseline")
