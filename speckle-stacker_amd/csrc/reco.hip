@@ -51,9 +51,23 @@ struct RecoArgs {
     int32_t X, Y;
     int32_t pmn0, pmn1, pmx0, pmx1; // PhaseMap signed range (Array2::range)
     int64_t nvisits;
+    int32_t push; // 1: windows whose schedule is static take the push chain
+    int32_t pad;
 };
 
 enum : int32_t { kPreU = 1, kPreV = 2, kValid = 4, kErr = 8 };
+
+// Per-visit status word E of the bulk pass: an eligible pre-window lookup was
+// out of range; the window cannot take the push chain (see reco_push_window).
+enum : int32_t { kEOutOfRange = 1, kEIrregular = 2 };
+
+// The push chain: a visit's terms whose younger in-window point was flagged at
+// most kPushAges visits before it are folded in by the chain itself, linearly
+// (one wavefront lane per pending visit); helpers prepare everything else from
+// the phases as they stood kPushLag visits before the visit.
+constexpr int kPushAges = 64;
+constexpr int kPushLag = 96;
+constexpr int kAgeNone = 1 << 30; // a point flagged before the window
 
 // c = conj(beta/|beta|) times the pre-window phase of whichever of u, v was
 // flagged before the window (records never have both); the fixup multiplies in
@@ -292,14 +306,39 @@ __device__ __forceinline__ void reco_map_body(int bx, const int2* __restrict__ v
     }
 }
 
-__device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const int2* __restrict__ visits, int64_t s,
-    int64_t win, const int64_t* __restrict__ map, const uint8_t* __restrict__ flags,
-    const double2* __restrict__ phases, const void* __restrict__ B, int is_f32, double2* __restrict__ S,
-    int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt, PendingRec* __restrict__ rec, int cap,
-    const int32_t* __restrict__ status)
+// A pending record as the push chain classifies it, for visit slot j: whether
+// it counts once the window's points are flagged at their first visits (every
+// in-window point is one whose calc_phase can flag it), and the ages (visits
+// between the flagging and visit j) of its two points, kAgeNone for a point
+// flagged before the window.
+__device__ inline bool push_counts(const RecoArgs& a, const PendingRec& r, int2 u, int2 v)
+{
+    return (r.f & kValid) && ((r.f & kPreU) || !calc_phase_returns_early(a, u))
+        && ((r.f & kPreV) || !calc_phase_returns_early(a, v));
+}
+__device__ inline void push_ages(const PendingRec& r, int j, int& young, int& old)
+{
+    const int au = (r.f & kPreU) ? kAgeNone : j - r.ju, av = (r.f & kPreV) ? kAgeNone : j - r.jv;
+    young = min(au, av);
+    old = max(au, av);
+}
+// The record fits the push chain: its younger point is at most kPushAges old and
+// the other one older than kPushLag (the chain adds it), or both are older than
+// kPushLag (a helper adds it).
+__device__ inline bool push_fits(int young, int old)
+{
+    return young <= kPushAges ? old > kPushLag : young > kPushLag;
+}
+
+__device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const int2* __restrict__ visits,
+    const int32_t* __restrict__ first_pos, int64_t s, int64_t win, const int64_t* __restrict__ map,
+    const uint8_t* __restrict__ flags, const double2* __restrict__ phases, const void* __restrict__ B, int is_f32,
+    double2* __restrict__ S, int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt,
+    PendingRec* __restrict__ rec, int cap, const int32_t* __restrict__ status)
 {
     __shared__ double red_re[kBulkThreads / 64], red_im[kBulkThreads / 64];
     __shared__ int red_n[kBulkThreads / 64], red_e[kBulkThreads / 64], red_p[kBulkThreads / 64];
+    __shared__ int push_irregular, push_counted;
     const int j = bx;
     const int tid = threadIdx.x;
     if (status[0])
@@ -313,6 +352,10 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
             E[j] = 0;
         }
         return;
+    }
+    if (tid == 0) {
+        push_irregular = 0;
+        push_counted = 0;
     }
     const Desc& d = a.d;
     const int vmn0 = d.min_idx[2], vmn1 = d.min_idx[3];
@@ -380,9 +423,10 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
     int base = incl - np;
     for (int wv = 0; wv < (tid >> 6); ++wv)
         base += red_p[wv];
+    int tn = 0, te = 0;
     if (tid == 0) {
         double tr = 0.0, ti = 0.0;
-        int tn = 0, te = 0, tp = 0;
+        int tp = 0;
         for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) {
             tr += red_re[wv];
             ti += red_im[wv];
@@ -392,10 +436,12 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
         }
         S[j] = mk<double2>(tr, ti);
         N[j] = tn;
-        E[j] = te;
         cnt[j] = tp;
     }
-    // pending records, in (thread, candidate) order
+    // pending records, in (thread, candidate) order; for the first visit of a
+    // point in the window, also whether the push chain can take it
+    const bool first_visit = first_pos[s + j] == j;
+    int irregular = 0, counted = 0;
     PendingRec* out = rec + (int64_t)j * cap + base;
     it = 0;
     for (int c = tid; c < ncand; c += blockDim.x, ++it) {
@@ -425,7 +471,27 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
                 r.c = fu ? cmul(phases[pu], bh) : (fv ? cmul(phases[pv], bh) : bh);
             }
         }
+        if (first_visit) {
+            if (r.f & kErr) {
+                irregular = 1;
+            } else if (push_counts(a, r, make_int2(ux, uy), make_int2(vx, vy))) {
+                int young, old;
+                push_ages(r, j, young, old);
+                ++counted;
+                irregular |= !push_fits(young, old);
+            }
+        }
         *out++ = r;
+    }
+    if (irregular)
+        atomicOr(&push_irregular, 1);
+    if (counted)
+        atomicAdd(&push_counted, counted);
+    __syncthreads();
+    if (tid == 0) {
+        // a first visit that would not flag breaks the push chain's static schedule too
+        const bool irr = first_visit && (push_irregular || tn + push_counted == 0);
+        E[j] = te | (irr ? kEIrregular : 0);
     }
 }
 
@@ -582,32 +648,303 @@ __device__ inline int side_state(bool pre, int32_t j, int32_t p, int32_t q, cons
 // visit k and finalises the point as calc_phase does.  Which record lands where
 // depends only on the visit list and on when each slot was flagged (ftime), so
 // the summation order, and every bit of the result, is the same on every run.
+// LDS of the fixup workgroup: the part both schedules use, then the general
+// schedule's state or the push chain's, which share memory.
+struct FixCommon {
+    StepInfo info[kWindow];
+    // phase and consistency of each window slot once flagged in this window
+    // (ppos x = kNotYet before: no unit or zero phase has x = 2); the window's
+    // results go to global memory once, at the end
+    double2 ppos[kWindow];
+    double pcons[kWindow];
+    int32_t progress, abort_flag, degenerate, pad;
+};
+struct FixGeneral {
+    int32_t ftime[kWindow]; // visit that flagged each slot
+    double2 far_s[kWindow];
+    // per visit: far count + (errors << 16), near count (-1: the chain takes
+    // every record of the visit), ready flag; written as one 16-byte store
+    // after far_s, read as one 16-byte load before it
+    i32x4 head[kWindow];
+    PendingRec nearbuf[kRing][kNearCap];
+};
+// A helper's result for one visit of the push chain: A = pre-window sum plus the
+// sum over the records whose points are both older than kPushLag visits, and
+// 1/n for the consistency.  call = 0: the visit runs no calc_phase.
+struct __align__(16) PushHead {
+    double2 A;
+    double inv_n;
+    int32_t call, ready;
+};
+struct FixPush {
+    PushHead head[kWindow];
+    // row k mod kPushLag holds Q_k[a - 1], a = 1 .. kPushAges
+    double2 Q[kPushLag * kPushAges];
+    int32_t own[kFixWaves][kPushAges]; // per helper: lowest lane adding to each age
+};
+struct FixLds {
+    FixCommon c;
+    union {
+        FixGeneral g;
+        FixPush p;
+    } u;
+};
+
+__device__ inline double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    return __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(b >> 32), l) << 32)
+        | (unsigned int)__builtin_amdgcn_readlane((int)(b & 0xffffffff), l));
+}
+
+// The push chain, for a window whose schedule is static: no visit of it has
+// kEIrregular, so every first visit of a point that calc_phase would run flags
+// that point, the visits that run are known in advance, and so is, for every
+// pending record, the visit at which each of its points is flagged.
+//
+// Wave 0 (the chain) keeps in lane k mod 64 the running sum of visit k, which it
+// finalises within the next 64 steps.  At step t it finalises visit t (lane
+// t mod 64: that sum plus the helper's part A_t), hands the lane to visit t + 64,
+// and adds phi_t Q_k[k - t] to the sum of every lane's visit k.  Q_k[a] is the
+// sum of the unit phasors X of visit k's records whose younger point was flagged
+// at visit k - a (a <= kPushAges) and whose other point is older than kPushLag
+// visits or was flagged before the window: the record's term is then
+// phi(younger) X, the reference's normalised product up to rounding, since both
+// factors are unit phasors.
+// Helper waves fill row Q_k and A_k (every record of visit k whose two points
+// are older than kPushLag visits) once the chain has finished visit
+// k - kPushLag - 1, so that the phases they read are final; they have
+// kPushLag - kPushAges steps of the chain until it first reads the row.
+// The step is a few FP64 operations and no reduction, so the serial path is
+// short.  A phase of zero (|mean| <= DBL_EPSILON, or NaN) makes the reference's
+// later terms with that point NaN (0/0), which a linear term would not
+// reproduce; the chain stops there and the window is redone by the general
+// schedule.  Returns false in that case and when a wait gave up.
+__device__ __forceinline__ bool reco_push_window(int nsteps, const PendingRec* __restrict__ rec, int cap,
+    int32_t* __restrict__ status, FixLds& L, int& calls, int& flagged)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    StepInfo* info = L.c.info;
+    double2* ppos = L.c.ppos;
+    double* pcons = L.c.pcons;
+    PushHead* head = L.u.p.head;
+    double2* Q = L.u.p.Q;
+    int32_t* progress = &L.c.progress;
+    int32_t* abort_flag = &L.c.abort_flag;
+    for (int t = tid; t < kWindow; t += kFixThreads)
+        head[t].ready = 0;
+    for (int t = tid; t < kFixWaves * kPushAges; t += kFixThreads)
+        L.u.p.own[t / kPushAges][t % kPushAges] = 64;
+    __syncthreads();
+    if (wave == 0) {
+        __builtin_amdgcn_s_setprio(3);
+        bool ok = true;
+        // rows and heads of visits 0 .. 64 are read at step 0
+        for (int k = 0; k <= min(kPushAges, nsteps - 1) && ok; ++k)
+            ok = wait_at_least(&head[k].ready, 1, abort_flag, status, lane);
+        if (ok) {
+            double zr = 0.0, zi = 0.0;
+            // step 0: lane 0 takes visit 64 (age 64), lane l > 0 holds visit l (age l)
+            int qi = lane == 0 ? (kPushAges % kPushLag) * kPushAges + kPushAges - 1 : lane * kPushAges + lane - 1;
+            double2 qv = Q[qi];
+            PushHead h = head[0];
+            int32_t rdy = kPushAges + 1 < nsteps
+                ? __hip_atomic_load(&head[kPushAges + 1].ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                : 1;
+            for (int t = 0; t < nsteps; ++t) {
+                const int call = __builtin_amdgcn_readfirstlane(h.call);
+                double pr = 0.0, pi = 0.0;
+                if (call) {
+                    const int sl = t & 63;
+                    const double tr = readlane_d(zr, sl) + h.A.x, ti = readlane_d(zi, sl) + h.A.y;
+                    const double m2 = tr * tr + ti * ti;
+                    const double ia = rsqrt_nr(m2);
+                    const double ap = m2 * ia * h.inv_n; // |sum| / n = |mean|
+                    if (__builtin_expect(!(ap > kEps), 0)) {
+                        if (lane == 0)
+                            L.c.degenerate = 1;
+                        lds_put(abort_flag, 1);
+                        ok = false;
+                        break;
+                    }
+                    pr = tr * ia;
+                    pi = ti * ia;
+                    if (lane == 0) {
+                        ppos[t] = mk<double2>(pr, pi);
+                        pcons[t] = ap;
+                    }
+                    ++calls;
+                    ++flagged;
+                }
+                if (lane == (t & 63)) { // the lane now collects visit t + 64
+                    zr = 0.0;
+                    zi = 0.0;
+                }
+                if (call) {
+                    zr = fma(pr, qv.x, fma(-pi, qv.y, zr));
+                    zi = fma(pr, qv.y, fma(pi, qv.x, zi));
+                }
+                if (t + 1 < nsteps) {
+                    // lane (t + 1) mod 64 takes visit kn at step t + 1: its row must be done
+                    const int kn = t + 1 + kPushAges;
+                    if (kn < nsteps && !__builtin_amdgcn_readfirstlane(rdy)) {
+                        if (!wait_at_least(&head[kn].ready, 1, abort_flag, status, lane)) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                    qi = lane == ((t + 1) & 63) ? (kn % kPushLag) * kPushAges + kPushAges - 1 : qi - 1;
+                    qv = Q[qi];
+                    h = head[t + 1];
+                    rdy = kn + 1 < nsteps
+                        ? __hip_atomic_load(&head[kn + 1].ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                        : 1;
+                }
+                if (lane == 0)
+                    lds_put(progress, t + 1);
+            }
+        }
+        if (!ok)
+            lds_put(abort_flag, 1);
+    } else {
+        // a helper: visits k = wave - 1 + i * kHelpers
+        int32_t* own = L.u.p.own[wave];
+        for (int k = wave - 1; k < nsteps; k += kHelpers) {
+            const int n0 = __builtin_amdgcn_readfirstlane(info[k].n0);
+            const int fp = __builtin_amdgcn_readfirstlane(info[k].fp);
+            PushHead* hk = &head[k];
+            if (n0 < 0 || fp != k) { // no calc_phase at this visit
+                if (lane == 0) {
+                    hk->A = mk<double2>(0.0, 0.0);
+                    hk->inv_n = 0.0;
+                    hk->call = 0;
+                    lds_put(&hk->ready, 1);
+                }
+                continue;
+            }
+            const int cn = __builtin_amdgcn_readfirstlane(info[k].cnt);
+            const PendingRec* rk = rec + (int64_t)k * cap;
+            PendingRec pre[2]; // the first 128 records travel while the helper waits
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                pre[c] = PendingRec {};
+                if (64 * c + lane < cn)
+                    pre[c] = rk[64 * c + lane];
+            }
+            const int p = k - kPushLag;
+            if (p > 0 && !wait_at_least(progress, p, abort_flag, status, lane))
+                break;
+            double2* row = Q + (k % kPushLag) * kPushAges;
+            row[lane] = mk<double2>(0.0, 0.0);
+            wave_lds_order();
+            double sr = 0.0, sim = 0.0;
+            int ne = 0;
+            for (int t0 = 0; t0 < cn; t0 += 64) {
+                const int t = t0 + lane;
+                PendingRec r;
+                if (t0 == 0)
+                    r = pre[0];
+                else if (t0 == 64)
+                    r = pre[1];
+                else
+                    r = t < cn ? rk[t] : PendingRec {};
+                // an in-window point runs calc_phase at its first visit (info n0 >= 0) or never
+                bool counts = false;
+                int young = 0, old = 0;
+                if (t < cn) {
+                    counts = (r.f & kValid) && ((r.f & kPreU) || info[max(r.ju, 0)].n0 >= 0)
+                        && ((r.f & kPreV) || info[max(r.jv, 0)].n0 >= 0);
+                    push_ages(r, k, young, old);
+                }
+                const bool far = counts && young > kPushLag;
+                const bool push = counts && young <= kPushAges;
+                double2 q = r.c;
+                if (far) {
+                    if (!(r.f & kPreU))
+                        q = cmul(ppos[r.ju], q);
+                    if (!(r.f & kPreV))
+                        q = cmul(q, ppos[r.jv]);
+                } else if (push && old != kAgeNone) {
+                    q = cmul(q, ppos[min(r.ju, r.jv)]); // the older in-window point
+                }
+                q = normalised(q);
+                if (far) {
+                    sr += q.x;
+                    sim += q.y;
+                }
+                ne += far || push;
+                // X into Q_k[young - 1]; records of one age add in lane order
+                bool pend = push;
+                while (__ballot(pend)) {
+                    if (pend)
+                        atomicMin(&own[young - 1], lane);
+                    wave_lds_order();
+                    const bool mine = pend && own[max(young, 1) - 1] == lane;
+                    wave_lds_order();
+                    if (mine) {
+                        const double2 cur = row[young - 1];
+                        row[young - 1] = mk<double2>(cur.x + q.x, cur.y + q.y);
+                        own[young - 1] = 64;
+                        pend = false;
+                    }
+                    wave_lds_order();
+                }
+            }
+            wave_sum3(ne, sr, sim);
+            if (lane == 0) {
+                const double2 S0 = info[k].S;
+                hk->A = mk<double2>(S0.x + sr, S0.y + sim);
+                hk->inv_n = 1.0 / (double)(n0 + ne);
+                hk->call = 1;
+                lds_put(&hk->ready, 1);
+            }
+        }
+    }
+    __syncthreads();
+    return !lds_get(abort_flag);
+}
+
+// The recursion over one window, by one workgroup of kFixWaves wavefronts: the
+// push chain above where the window's schedule is static, else (and where the
+// push chain met a zero phase) the general schedule below.
+//
+// General schedule.  Wave 0 (the chain) walks the visits in order.  For visit k
+// it needs the sum over the visit's pending records whose points are flagged by
+// now.  Helper waves take visits k = h, h + kHelpers, ... ahead of the chain:
+// once the chain has finished visit p - 1 (p = k - kLead), a record both of whose
+// in-window points were flagged before p is certain to count, and one with a
+// point neither flagged before p nor visited in [p, k) is certain not to; the
+// helper sums the first kind into far[k] and copies only the undecided rest
+// ("near" records) to LDS.  The chain then adds the few near records whose
+// points are flagged at visit k and finalises the point as calc_phase does.
+// Which record lands where depends only on the visit list and on when each slot
+// was flagged (ftime), so the summation order, and every bit of the result, is
+// the same on every run.
 __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
     double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
-    __shared__ StepInfo info[kWindow];
-    // phase, consistency and flagging visit of each window slot once flagged in
-    // this window (ppos x = kNotYet before: no unit or zero phase has x = 2).
-    // The window's results go to global memory once, at the end.
-    __shared__ double2 ppos[kWindow];
-    __shared__ double pcons[kWindow];
-    __shared__ int32_t ftime[kWindow];
-    __shared__ double2 far_s[kWindow];
-    // per visit: far count + (errors << 16), near count (-1: the chain takes
-    // every record of the visit), ready flag; written as one 16-byte store
-    // after far_s, read as one 16-byte load before it
-    __shared__ i32x4 head[kWindow];
-    __shared__ PendingRec nearbuf[kRing][kNearCap];
-    __shared__ int32_t progress, abort_flag;
+    __shared__ __align__(16) unsigned char fix_lds[sizeof(FixLds)];
+    FixLds& L = *reinterpret_cast<FixLds*>(fix_lds);
+    StepInfo* info = L.c.info;
+    double2* ppos = L.c.ppos;
+    double* pcons = L.c.pcons;
+    int32_t* ftime = L.u.g.ftime;
+    double2* far_s = L.u.g.far_s;
+    i32x4* head = L.u.g.head;
+    PendingRec(*nearbuf)[kNearCap] = L.u.g.nearbuf;
+    int32_t& progress = L.c.progress;
+    int32_t& abort_flag = L.c.abort_flag;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nsteps = (int)(e - s);
     if (tid == 0) {
         progress = 0;
         abort_flag = status[0] != 0; // an earlier window raised
+        L.c.degenerate = 0;
     }
+    bool irregular = false;
     for (int t = tid; t < nsteps; t += kFixThreads) {
         StepInfo si;
         si.n0 = N[t];
@@ -616,16 +953,35 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
         si.err = E[t];
         si.S = S[t];
         info[t] = si;
+        irregular |= si.err != 0;
     }
-    for (int t = tid; t < kWindow; t += kFixThreads) {
+    for (int t = tid; t < kWindow; t += kFixThreads)
         ppos[t] = mk<double2>(kNotYet, 0.0);
-        ftime[t] = kNever;
-        head[t] = i32x4 { 0, 0, 0, 0 };
-    }
-    __syncthreads();
+    irregular = __syncthreads_or(irregular) != 0;
 
     int calls = 0, flagged = 0;
-    if (!lds_get(&abort_flag)) {
+    bool done = false;
+    if (a.push && !irregular && !lds_get(&abort_flag)) {
+        done = reco_push_window(nsteps, rec, cap, status, L, calls, flagged);
+        if (!done && L.c.degenerate && !status[0]) { // redo the window by the general schedule
+            for (int t = tid; t < kWindow; t += kFixThreads)
+                ppos[t] = mk<double2>(kNotYet, 0.0);
+            calls = flagged = 0;
+            __syncthreads();
+            if (tid == 0) {
+                progress = 0;
+                abort_flag = 0;
+            }
+        }
+    }
+    if (!done) {
+        for (int t = tid; t < kWindow; t += kFixThreads) {
+            ftime[t] = kNever;
+            head[t] = i32x4 { 0, 0, 0, 0 };
+        }
+        __syncthreads();
+    }
+    if (!done && !lds_get(&abort_flag)) {
         auto take = [&](const PendingRec& r, double2 pu, double2 pv, double& sr, double& sim, int& ne) {
             const bool uok = (r.f & kPreU) || (r.ju >= 0 && pu.x != kNotYet);
             const bool vok = (r.f & kPreV) || (r.jv >= 0 && pv.x != kNotYet);
@@ -756,7 +1112,7 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
                         nerr = ne >> 16;
                     }
                     const int fn = __builtin_amdgcn_readfirstlane(A.hd.x);
-                    if (__builtin_expect((fn >> 16) || nerr || __builtin_amdgcn_readfirstlane(A.si.err), 0)) {
+                    if (__builtin_expect((fn >> 16) || nerr || (__builtin_amdgcn_readfirstlane(A.si.err) & kEOutOfRange), 0)) {
                         if (lane == 0)
                             status[0] = SMIP_ERR_OUT_OF_RANGE; // get_element throws std::out_of_range
                         lds_put(&abort_flag, 1);
@@ -898,13 +1254,14 @@ __global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __res
     reco_map_body(blockIdx.x, visits, first_pos, a, s, e, win, map);
 }
 
-__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits, int64_t s,
-    int64_t win, const int64_t* __restrict__ map, const uint8_t* __restrict__ flags,
-    const double2* __restrict__ phases, const void* __restrict__ B, int is_f32, double2* __restrict__ S,
-    int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt, PendingRec* __restrict__ rec, int cap,
-    const int32_t* __restrict__ status)
+__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits,
+    const int32_t* __restrict__ first_pos, int64_t s, int64_t win, const int64_t* __restrict__ map,
+    const uint8_t* __restrict__ flags, const double2* __restrict__ phases, const void* __restrict__ B, int is_f32,
+    double2* __restrict__ S, int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt,
+    PendingRec* __restrict__ rec, int cap, const int32_t* __restrict__ status)
 {
-    reco_bulk_body(blockIdx.x, a, visits, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec, cap, status);
+    reco_bulk_body(blockIdx.x, a, visits, first_pos, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec, cap,
+        status);
 }
 
 __global__ __launch_bounds__(kFixThreads) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
@@ -954,7 +1311,8 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* 
     const int64_t s0 = win * kWindow;
     if (s0 + (int64_t)blockIdx.x >= a.nvisits)
         return;
-    reco_bulk_body(blockIdx.x, a, global_ptr(J.visits), s0, win, global_ptr(J.map), global_ptr(J.flags),
+    reco_bulk_body(blockIdx.x, a, global_ptr(J.visits), global_ptr(J.first_pos), s0, win, global_ptr(J.map),
+        global_ptr(J.flags),
         global_ptr(J.phases), global_ptr(J.B), J.is_f32, global_ptr(J.S), global_ptr(J.N), global_ptr(J.E),
         global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
 }
@@ -1090,6 +1448,13 @@ static RecoArgs reco_args(const Desc& d, int32_t xsize, int32_t ysize, int64_t n
     a.pmx0 = -xsize / 2 + xsize - 1;
     a.pmx1 = -ysize / 2 + ysize - 1;
     a.nvisits = nvisits;
+    // SMIP_RECO_PUSH=0 keeps every window on the general schedule (a cross-check)
+    static const bool push = [] {
+        const char* v = std::getenv("SMIP_RECO_PUSH");
+        return !(v && v[0] == '0');
+    }();
+    a.push = push ? 1 : 0;
+    a.pad = 0;
     return a;
 }
 
@@ -1292,8 +1657,8 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
             if (win == 0) // later windows' maps are written by the fixup before them
                 hipLaunchKernelGGL(k_reco_map, dim3((nk + 255) / 256), dim3(256), 0, s, w->visits, w->first_pos, a,
                     s0, e0, win, w->map);
-            hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, s0, win, w->map, flags,
-                phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
+            hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, w->first_pos, s0, win,
+                w->map, flags, phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
             hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(kFixThreads), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
                 w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status, win, w->map);
         }
