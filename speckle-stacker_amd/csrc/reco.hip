@@ -35,6 +35,7 @@
 #include <memory>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "smip_complex.h"
@@ -257,6 +258,14 @@ __device__ inline double rsqrt_nr(double x)
     }
     return y;
 }
+// the same with one Newton-Raphson step: about 2^-44 relative; the push chain
+// scales a phasor by it, which leaves its direction exact to rounding
+__device__ inline double rsqrt_nr1(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(0.5 * y, e, y);
+}
 __device__ inline double rcp_nr(double x)
 {
     double y = __builtin_amdgcn_rcp(x);
@@ -330,6 +339,7 @@ __device__ inline bool push_fits(int young, int old)
     return young <= kPushAges ? old > kPushLag : young > kPushLag;
 }
 
+template <bool PUSH>
 __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t win, const int64_t* __restrict__ map,
     const uint8_t* __restrict__ flags, const double2* __restrict__ phases, const void* __restrict__ B, int is_f32,
@@ -353,7 +363,7 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
         }
         return;
     }
-    if (tid == 0) {
+    if (PUSH && tid == 0) {
         push_irregular = 0;
         push_counted = 0;
     }
@@ -437,6 +447,8 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
         S[j] = mk<double2>(tr, ti);
         N[j] = tn;
         cnt[j] = tp;
+        if (!PUSH)
+            E[j] = te;
     }
     // pending records, in (thread, candidate) order; for the first visit of a
     // point in the window, also whether the push chain can take it
@@ -471,7 +483,7 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
                 r.c = fu ? cmul(phases[pu], bh) : (fv ? cmul(phases[pv], bh) : bh);
             }
         }
-        if (first_visit) {
+        if (PUSH && first_visit) {
             if (r.f & kErr) {
                 irregular = 1;
             } else if (push_counts(a, r, make_int2(ux, uy), make_int2(vx, vy))) {
@@ -483,15 +495,17 @@ __device__ __forceinline__ void reco_bulk_body(int bx, const RecoArgs& a, const 
         }
         *out++ = r;
     }
-    if (irregular)
-        atomicOr(&push_irregular, 1);
-    if (counted)
-        atomicAdd(&push_counted, counted);
-    __syncthreads();
-    if (tid == 0) {
-        // a first visit that would not flag breaks the push chain's static schedule too
-        const bool irr = first_visit && (push_irregular || tn + push_counted == 0);
-        E[j] = te | (irr ? kEIrregular : 0);
+    if (PUSH) {
+        if (irregular)
+            atomicOr(&push_irregular, 1);
+        if (counted)
+            atomicAdd(&push_counted, counted);
+        __syncthreads();
+        if (tid == 0) {
+            // a first visit that would not flag breaks the push chain's static schedule too
+            const bool irr = first_visit && (push_irregular || tn + push_counted == 0);
+            E[j] = te | (irr ? kEIrregular : 0);
+        }
     }
 }
 
@@ -508,9 +522,13 @@ typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 typedef int32_t i32x3 __attribute__((ext_vector_type(3)));
 
 constexpr double kNotYet = 2.0;           // ppos marker: slot not flagged in this window
-constexpr int kFixWaves = 8;              // the chain wave and seven helpers
-constexpr int kFixThreads = 64 * kFixWaves;
-constexpr int kHelpers = kFixWaves - 1;
+// Waves of a fixup workgroup: the chain wave and its helpers.  The push chain's
+// helpers have more to do per visit than the general schedule's.
+constexpr int kPushWaves = 12;
+constexpr int kPushThreads = 64 * kPushWaves;
+constexpr int kPushHelpers = kPushWaves - 1;
+constexpr int kGenWaves = 8;
+constexpr int kGenThreads = 64 * kGenWaves;
 constexpr int kLead = 8;                  // visits between a helper's snapshot and its visit
 constexpr int kRing = 16;                 // near-record slots, more than kLead
 constexpr int kNearCap = 32;              // near records per visit; more: the chain takes every record
@@ -635,19 +653,6 @@ __device__ inline int side_state(bool pre, int32_t j, int32_t p, int32_t q, cons
     return seen ? 1 : 2;
 }
 
-// The recursion over one window, by one workgroup of kFixWaves wavefronts.
-//
-// Wave 0 (the chain) walks the visits in order.  For visit k it needs the sum
-// over the visit's pending records whose points are flagged by now.  Helper
-// waves take visits k = h, h + kHelpers, ... ahead of the chain: once the chain
-// has finished visit p - 1 (p = k - kLead), a record both of whose in-window
-// points were flagged before p is certain to count, and one with a point neither
-// flagged before p nor visited in [p, k) is certain not to; the helper sums the
-// first kind into far[k] and copies only the undecided rest ("near" records) to
-// LDS.  The chain then adds the few near records whose points are flagged at
-// visit k and finalises the point as calc_phase does.  Which record lands where
-// depends only on the visit list and on when each slot was flagged (ftime), so
-// the summation order, and every bit of the result, is the same on every run.
 // LDS of the fixup workgroup: the part both schedules use, then the general
 // schedule's state or the push chain's, which share memory.
 struct FixCommon {
@@ -669,24 +674,40 @@ struct FixGeneral {
     PendingRec nearbuf[kRing][kNearCap];
 };
 // A helper's result for one visit of the push chain: A = pre-window sum plus the
-// sum over the records whose points are both older than kPushLag visits, and
-// 1/n for the consistency.  call = 0: the visit runs no calc_phase.
+// sum over the records whose points are both older than kPushLag visits; the
+// eligible count n and thr = (DBL_EPSILON n)^2, below which |sum|^2 means a
+// zero phase.  call = 0: the visit runs no calc_phase.
 struct __align__(16) PushHead {
     double2 A;
-    double inv_n;
-    int32_t call, ready;
+    double thr;
+    int32_t call, n;
 };
+// Q_k[a] lives in row (k - a) mod kPushLag, the step that adds it, at column
+// k mod 64, the lane that holds visit k: the chain reads one contiguous row per
+// step.  Rows are padded by one entry so that a helper's 64 entries of one
+// visit, one per row, fall in different banks.
+constexpr int kPushRow = kPushAges + 1;
+__device__ inline int push_slot(int k, int a) { return ((k - a) % kPushLag) * kPushRow + (k & 63); }
+constexpr int kReadyWords = kWindow + kPushAges + 8;
 struct FixPush {
-    PushHead head[kWindow];
-    // row k mod kPushLag holds Q_k[a - 1], a = 1 .. kPushAges
-    double2 Q[kPushLag * kPushAges];
-    int32_t own[kFixWaves][kPushAges]; // per helper: lowest lane adding to each age
+    PushHead head[kWindow + 2]; // + the step that pads an odd window
+    // ready[k + 1]: head[k] and the entries Q_k are done (1 beyond the window)
+    int32_t ready[kReadyWords];
+    double2 Q[kPushLag * kPushRow];
+    int32_t ncalls;
 };
 struct FixLds {
     FixCommon c;
     union {
         FixGeneral g;
         FixPush p;
+    } u;
+};
+// the general schedule alone (grouped recoveries)
+struct FixLdsGeneral {
+    FixCommon c;
+    union {
+        FixGeneral g;
     } u;
 };
 
@@ -710,132 +731,168 @@ __device__ inline double readlane_d(double v, int l)
 // at visit k - a (a <= kPushAges) and whose other point is older than kPushLag
 // visits or was flagged before the window: the record's term is then
 // phi(younger) X, the reference's normalised product up to rounding, since both
-// factors are unit phasors.
-// Helper waves fill row Q_k and A_k (every record of visit k whose two points
-// are older than kPushLag visits) once the chain has finished visit
-// k - kPushLag - 1, so that the phases they read are final; they have
-// kPushLag - kPushAges steps of the chain until it first reads the row.
+// factors are unit phasors.  At most two records share an entry (the same two
+// points as (u, v) and as (v, u)), so adding them into a zeroed entry with LDS
+// atomics gives the same bits in either order.
+// Helper waves fill the entries Q_k and A_k (every record of visit k whose two
+// points are older than kPushLag visits) once the chain has finished visit
+// k - kPushLag - 1, so that the phases they read are final; they have about
+// kPushLag - kPushAges steps of the chain until it first reads an entry.
 // The step is a few FP64 operations and no reduction, so the serial path is
 // short.  A phase of zero (|mean| <= DBL_EPSILON, or NaN) makes the reference's
 // later terms with that point NaN (0/0), which a linear term would not
-// reproduce; the chain stops there and the window is redone by the general
-// schedule.  Returns false in that case and when a wait gave up.
+// reproduce; the window is then redone by the general schedule.  Returns false
+// in that case and when a wait gave up.
 __device__ __forceinline__ bool reco_push_window(int nsteps, const PendingRec* __restrict__ rec, int cap,
     int32_t* __restrict__ status, FixLds& L, int& calls, int& flagged)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     StepInfo* info = L.c.info;
     double2* ppos = L.c.ppos;
-    double* pcons = L.c.pcons;
+    double* pcons = L.c.pcons; // |sum|^2 until the end of the window, then the consistency
     PushHead* head = L.u.p.head;
     double2* Q = L.u.p.Q;
+    int32_t* ready = L.u.p.ready;
     int32_t* progress = &L.c.progress;
     int32_t* abort_flag = &L.c.abort_flag;
-    for (int t = tid; t < kWindow; t += kFixThreads)
-        head[t].ready = 0;
-    for (int t = tid; t < kFixWaves * kPushAges; t += kFixThreads)
-        L.u.p.own[t / kPushAges][t % kPushAges] = 64;
+    for (int t = tid; t < kReadyWords; t += kPushThreads)
+        ready[t] = t - 1 >= nsteps ? 1 : 0;
+    if (tid < 2) { // the pad step of an odd window, and the one after it
+        head[nsteps + tid].A = mk<double2>(0.0, 0.0);
+        head[nsteps + tid].thr = 0.0;
+        head[nsteps + tid].call = 0;
+        head[nsteps + tid].n = 0;
+    }
+    if (tid == 0)
+        L.u.p.ncalls = 0;
     __syncthreads();
     if (wave == 0) {
         __builtin_amdgcn_s_setprio(3);
         bool ok = true;
-        // rows and heads of visits 0 .. 64 are read at step 0
+        // the entries and heads of visits 0 .. 64 are read at step 0
         for (int k = 0; k <= min(kPushAges, nsteps - 1) && ok; ++k)
-            ok = wait_at_least(&head[k].ready, 1, abort_flag, status, lane);
+            ok = wait_at_least(&ready[k + 1], 1, abort_flag, status, lane);
         if (ok) {
             double zr = 0.0, zi = 0.0;
-            // step 0: lane 0 takes visit 64 (age 64), lane l > 0 holds visit l (age l)
-            int qi = lane == 0 ? (kPushAges % kPushLag) * kPushAges + kPushAges - 1 : lane * kPushAges + lane - 1;
-            double2 qv = Q[qi];
-            PushHead h = head[0];
-            int32_t rdy = kPushAges + 1 < nsteps
-                ? __hip_atomic_load(&head[kPushAges + 1].ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                : 1;
-            for (int t = 0; t < nsteps; ++t) {
-                const int call = __builtin_amdgcn_readfirstlane(h.call);
-                double pr = 0.0, pi = 0.0;
-                if (call) {
-                    const int sl = t & 63;
-                    const double tr = readlane_d(zr, sl) + h.A.x, ti = readlane_d(zi, sl) + h.A.y;
-                    const double m2 = tr * tr + ti * ti;
-                    const double ia = rsqrt_nr(m2);
-                    const double ap = m2 * ia * h.inv_n; // |sum| / n = |mean|
-                    if (__builtin_expect(!(ap > kEps), 0)) {
-                        if (lane == 0)
-                            L.c.degenerate = 1;
-                        lds_put(abort_flag, 1);
-                        ok = false;
-                        break;
-                    }
-                    pr = tr * ia;
-                    pi = ti * ia;
-                    if (lane == 0) {
+            // What a step reads, loaded one step ahead: this lane's Q entry and the
+            // visit's helper result.
+            struct Ahead {
+                double2 qv;
+                double2 A;
+                double thr;
+                int32_t call;
+            };
+            bool bad = false; // a zero phase met (checked once, after the loop)
+            int row1 = 0;     // (t + 1) mod kPushLag
+            // One step: loads of step t + 1 (whose entries were checked ready),
+            // then visit t (lane t mod 64) finalised and phi_t added to every lane.
+            auto step = [&](int t, const Ahead& cur, Ahead& nxt, bool odd) {
+                const int t1 = t + 1;
+                row1 = row1 + 1 == kPushLag ? 0 : row1 + 1;
+                nxt.qv = Q[row1 * kPushRow + lane];
+                const PushHead* hn = &head[t1];
+                nxt.A = hn->A;
+                nxt.thr = hn->thr;
+                nxt.call = hn->call;
+                const bool call = __builtin_amdgcn_readfirstlane(cur.call) != 0;
+                const int sl = t & 63;
+                const double tr = readlane_d(zr, sl) + cur.A.x, ti = readlane_d(zi, sl) + cur.A.y;
+                // a visit without a calc_phase has a zero sum: the 1e-300 keeps its
+                // factor finite, so that it pushes exact zeros
+                const double m2 = fma(ti, ti, fma(tr, tr, 1e-300));
+                const double ia = rsqrt_nr1(m2);
+                bad |= call && !(m2 > cur.thr);
+                const double pr = tr * ia, pi = ti * ia;
+                if (lane == 0) {
+                    if (call) {
                         ppos[t] = mk<double2>(pr, pi);
-                        pcons[t] = ap;
+                        pcons[t] = m2;
                     }
-                    ++calls;
-                    ++flagged;
+                    if (odd)
+                        lds_put(progress, t1);
                 }
-                if (lane == (t & 63)) { // the lane now collects visit t + 64
-                    zr = 0.0;
-                    zi = 0.0;
-                }
-                if (call) {
-                    zr = fma(pr, qv.x, fma(-pi, qv.y, zr));
-                    zi = fma(pr, qv.y, fma(pi, qv.x, zi));
-                }
-                if (t + 1 < nsteps) {
-                    // lane (t + 1) mod 64 takes visit kn at step t + 1: its row must be done
-                    const int kn = t + 1 + kPushAges;
-                    if (kn < nsteps && !__builtin_amdgcn_readfirstlane(rdy)) {
-                        if (!wait_at_least(&head[kn].ready, 1, abort_flag, status, lane)) {
+                // lane t mod 64 now collects visit t + 64; its entry at this step is
+                // Q_{t+64}[64].  A sum that is not finite only ever belongs to a
+                // window that is redone.
+                const double keep = lane == sl ? 0.0 : 1.0;
+                zr = fma(pr, cur.qv.x, fma(-pi, cur.qv.y, zr * keep));
+                zi = fma(pr, cur.qv.y, fma(pi, cur.qv.x, zi * keep));
+            };
+            Ahead A, B;
+            A.qv = Q[lane];
+            A.A = head[0].A;
+            A.thr = head[0].thr;
+            A.call = head[0].call;
+            // two steps per iteration, so that the two prefetch sets swap roles
+            // instead of being copied (an odd window ends with a pad step that
+            // pushes only into visits beyond it); before steps t + 1 and t + 2 load
+            // the entries of visits t + 65 and t + 66, one check that both are done
+            int2 rdy = *reinterpret_cast<const int2*>(&ready[kPushAges + 2]);
+            for (int t = 0; t < nsteps; t += 2) {
+                if (__builtin_expect(!__builtin_amdgcn_readfirstlane(rdy.x & rdy.y), 0)) {
+                    for (int k = t + kPushAges + 1; k <= t + kPushAges + 2; ++k)
+                        if (!wait_at_least(&ready[k + 1], 1, abort_flag, status, lane))
                             ok = false;
-                            break;
-                        }
-                    }
-                    qi = lane == ((t + 1) & 63) ? (kn % kPushLag) * kPushAges + kPushAges - 1 : qi - 1;
-                    qv = Q[qi];
-                    h = head[t + 1];
-                    rdy = kn + 1 < nsteps
-                        ? __hip_atomic_load(&head[kn + 1].ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                        : 1;
                 }
+                rdy = *reinterpret_cast<const int2*>(&ready[t + kPushAges + 4]);
+                step(t, A, B, false);
+                step(t + 1, B, A, true);
+            }
+            if (lane == 0)
+                lds_put(progress, nsteps);
+            if (bad) {
                 if (lane == 0)
-                    lds_put(progress, t + 1);
+                    L.c.degenerate = 1;
+                ok = false;
             }
         }
         if (!ok)
             lds_put(abort_flag, 1);
     } else {
-        // a helper: visits k = wave - 1 + i * kHelpers
-        int32_t* own = L.u.p.own[wave];
-        for (int k = wave - 1; k < nsteps; k += kHelpers) {
-            const int n0 = __builtin_amdgcn_readfirstlane(info[k].n0);
-            const int fp = __builtin_amdgcn_readfirstlane(info[k].fp);
-            PushHead* hk = &head[k];
-            if (n0 < 0 || fp != k) { // no calc_phase at this visit
-                if (lane == 0) {
-                    hk->A = mk<double2>(0.0, 0.0);
-                    hk->inv_n = 0.0;
-                    hk->call = 0;
-                    lds_put(&hk->ready, 1);
-                }
-                continue;
-            }
-            const int cn = __builtin_amdgcn_readfirstlane(info[k].cnt);
+        // a helper: visits k = wave - 1 + i * kPushHelpers.  The first 128 records of
+        // its next visit are loaded while it works on the current one.
+        PendingRec pre[2];
+        auto fetch = [&](int k) {
+            const int cn = k < nsteps ? __builtin_amdgcn_readfirstlane(info[k].cnt) : 0;
             const PendingRec* rk = rec + (int64_t)k * cap;
-            PendingRec pre[2]; // the first 128 records travel while the helper waits
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 pre[c] = PendingRec {};
                 if (64 * c + lane < cn)
                     pre[c] = rk[64 * c + lane];
             }
-            const int p = k - kPushLag;
+        };
+        int ncalls = 0;
+        fetch(wave - 1);
+        for (int k = wave - 1; k < nsteps; k += kPushHelpers) {
+            const int n0 = __builtin_amdgcn_readfirstlane(info[k].n0);
+            const int fp = __builtin_amdgcn_readfirstlane(info[k].fp);
+            PushHead* hk = &head[k];
+            const int p = k - kPushLag; // the chain has finished visit p - 1: the entries' rows are free
+            if (n0 < 0 || fp != k) { // no calc_phase at this visit: zero entries
+                fetch(k + kPushHelpers);
+                if (p > 0 && !wait_at_least(progress, p, abort_flag, status, lane))
+                    break;
+                if (lane < k) // entries of steps k - 64 .. k - 1 that exist
+                    Q[push_slot(k, lane + 1)] = mk<double2>(0.0, 0.0);
+                if (lane == 0) {
+                    hk->A = mk<double2>(0.0, 0.0);
+                    hk->thr = 0.0;
+                    hk->call = 0;
+                    hk->n = 0;
+                    lds_put(&ready[k + 1], 1);
+                }
+                continue;
+            }
+            ++ncalls;
+            const int cn = __builtin_amdgcn_readfirstlane(info[k].cnt);
+            const PendingRec* rk = rec + (int64_t)k * cap;
+            const PendingRec cur0 = pre[0], cur1 = pre[1];
+            fetch(k + kPushHelpers);
             if (p > 0 && !wait_at_least(progress, p, abort_flag, status, lane))
                 break;
-            double2* row = Q + (k % kPushLag) * kPushAges;
-            row[lane] = mk<double2>(0.0, 0.0);
+            if (lane < k)
+                Q[push_slot(k, lane + 1)] = mk<double2>(0.0, 0.0);
             wave_lds_order();
             double sr = 0.0, sim = 0.0;
             int ne = 0;
@@ -843,9 +900,9 @@ __device__ __forceinline__ bool reco_push_window(int nsteps, const PendingRec* _
                 const int t = t0 + lane;
                 PendingRec r;
                 if (t0 == 0)
-                    r = pre[0];
+                    r = cur0;
                 else if (t0 == 64)
-                    r = pre[1];
+                    r = cur1;
                 else
                     r = t < cn ? rk[t] : PendingRec {};
                 // an in-window point runs calc_phase at its first visit (info n0 >= 0) or never
@@ -873,40 +930,42 @@ __device__ __forceinline__ bool reco_push_window(int nsteps, const PendingRec* _
                     sim += q.y;
                 }
                 ne += far || push;
-                // X into Q_k[young - 1]; records of one age add in lane order
-                bool pend = push;
-                while (__ballot(pend)) {
-                    if (pend)
-                        atomicMin(&own[young - 1], lane);
-                    wave_lds_order();
-                    const bool mine = pend && own[max(young, 1) - 1] == lane;
-                    wave_lds_order();
-                    if (mine) {
-                        const double2 cur = row[young - 1];
-                        row[young - 1] = mk<double2>(cur.x + q.x, cur.y + q.y);
-                        own[young - 1] = 64;
-                        pend = false;
-                    }
-                    wave_lds_order();
+                if (push) { // X into Q_k[young]
+                    double2* e = &Q[push_slot(k, young)];
+                    __hip_atomic_fetch_add(&e->x, q.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&e->y, q.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
             wave_sum3(ne, sr, sim);
+            wave_lds_order(); // the entries' atomics and the head before the ready word
             if (lane == 0) {
                 const double2 S0 = info[k].S;
+                const int n = n0 + ne;
                 hk->A = mk<double2>(S0.x + sr, S0.y + sim);
-                hk->inv_n = 1.0 / (double)(n0 + ne);
+                hk->thr = (kEps * n) * (kEps * n);
                 hk->call = 1;
-                lds_put(&hk->ready, 1);
+                hk->n = n;
+                lds_put(&ready[k + 1], 1);
             }
         }
+        if (lane == 0)
+            atomicAdd(&L.u.p.ncalls, ncalls);
     }
     __syncthreads();
-    return !lds_get(abort_flag);
+    const bool ok = !lds_get(abort_flag);
+    if (ok) { // consistency |sum| / n of each flagged slot
+        for (int t = tid; t < nsteps; t += kPushThreads)
+            if (ppos[t].x != kNotYet)
+                pcons[t] = sqrt(pcons[t]) / (double)head[t].n;
+        calls = flagged = L.u.p.ncalls;
+    }
+    return ok;
 }
 
-// The recursion over one window, by one workgroup of kFixWaves wavefronts: the
-// push chain above where the window's schedule is static, else (and where the
-// push chain met a zero phase) the general schedule below.
+// The recursion over one window, by one workgroup: with PUSH (one recovery at a
+// time), the push chain above where the window's schedule is static, else (and
+// where the push chain met a zero phase) the general schedule below; without
+// PUSH (grouped recoveries), the general schedule alone.
 //
 // General schedule.  Wave 0 (the chain) walks the visits in order.  For visit k
 // it needs the sum over the visit's pending records whose points are flagged by
@@ -920,14 +979,18 @@ __device__ __forceinline__ bool reco_push_window(int nsteps, const PendingRec* _
 // Which record lands where depends only on the visit list and on when each slot
 // was flagged (ftime), so the summation order, and every bit of the result, is
 // the same on every run.
+template <bool PUSH>
 __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
     double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
-    __shared__ __align__(16) unsigned char fix_lds[sizeof(FixLds)];
-    FixLds& L = *reinterpret_cast<FixLds*>(fix_lds);
+    using Lds = typename std::conditional<PUSH, FixLds, FixLdsGeneral>::type;
+    constexpr int kFixThreads = PUSH ? kPushThreads : kGenThreads;
+    constexpr int kHelpers = kFixThreads / 64 - 1;
+    __shared__ __align__(16) unsigned char fix_lds[sizeof(Lds)];
+    Lds& L = *reinterpret_cast<Lds*>(fix_lds);
     StepInfo* info = L.c.info;
     double2* ppos = L.c.ppos;
     double* pcons = L.c.pcons;
@@ -961,8 +1024,9 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
 
     int calls = 0, flagged = 0;
     bool done = false;
-    if (a.push && !irregular && !lds_get(&abort_flag)) {
-        done = reco_push_window(nsteps, rec, cap, status, L, calls, flagged);
+    if constexpr (PUSH) {
+        if (a.push && !irregular && !lds_get(&abort_flag)) {
+            done = reco_push_window(nsteps, rec, cap, status, L, calls, flagged);
         if (!done && L.c.degenerate && !status[0]) { // redo the window by the general schedule
             for (int t = tid; t < kWindow; t += kFixThreads)
                 ppos[t] = mk<double2>(kNotYet, 0.0);
@@ -972,6 +1036,7 @@ __device__ __forceinline__ void reco_fixup_body(const RecoArgs& a, const int2* _
                 progress = 0;
                 abort_flag = 0;
             }
+        }
         }
     }
     if (!done) {
@@ -1260,17 +1325,17 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const in
     double2* __restrict__ S, int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt,
     PendingRec* __restrict__ rec, int cap, const int32_t* __restrict__ status)
 {
-    reco_bulk_body(blockIdx.x, a, visits, first_pos, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec, cap,
-        status);
+    reco_bulk_body<true>(blockIdx.x, a, visits, first_pos, s, win, map, flags, phases, B, is_f32, S, N, E, cnt, rec,
+        cap, status);
 }
 
-__global__ __launch_bounds__(kFixThreads) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
+__global__ __launch_bounds__(kPushThreads) void k_reco_fixup(RecoArgs a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t e, const double2* __restrict__ S,
     const int32_t* __restrict__ N, const int32_t* __restrict__ E, const int32_t* __restrict__ cnt,
     const PendingRec* __restrict__ rec, int cap, uint8_t* __restrict__ flags, double2* __restrict__ phases,
     double* __restrict__ cons, int32_t* __restrict__ status, int64_t win, int64_t* __restrict__ map)
 {
-    reco_fixup_body(a, visits, first_pos, s, e, S, N, E, cnt, rec, cap, flags, phases, cons, status, win, map);
+    reco_fixup_body<true>(a, visits, first_pos, s, e, S, N, E, cnt, rec, cap, flags, phases, cons, status, win, map);
 }
 
 // ---------------------------------------------------------------------------
@@ -1311,20 +1376,21 @@ __global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* 
     const int64_t s0 = win * kWindow;
     if (s0 + (int64_t)blockIdx.x >= a.nvisits)
         return;
-    reco_bulk_body(blockIdx.x, a, global_ptr(J.visits), global_ptr(J.first_pos), s0, win, global_ptr(J.map),
+    reco_bulk_body<false>(blockIdx.x, a, global_ptr(J.visits), global_ptr(J.first_pos), s0, win, global_ptr(J.map),
         global_ptr(J.flags),
         global_ptr(J.phases), global_ptr(J.B), J.is_f32, global_ptr(J.S), global_ptr(J.N), global_ptr(J.E),
         global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
 }
 
-__global__ __launch_bounds__(kFixThreads) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
+__global__ __launch_bounds__(kGenThreads) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
 {
     const RecoJob& J = jobs[blockIdx.x];
     const RecoArgs a = J.a;
     const int64_t s0 = win * kWindow;
     if (s0 >= a.nvisits)
         return;
-    reco_fixup_body(a, global_ptr(J.visits), global_ptr(J.first_pos), s0, min(s0 + kWindow, a.nvisits), global_ptr(J.S),
+    reco_fixup_body<false>(a, global_ptr(J.visits), global_ptr(J.first_pos), s0, min(s0 + kWindow, a.nvisits),
+        global_ptr(J.S),
         global_ptr(J.N), global_ptr(J.E), global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.flags),
         global_ptr(J.phases), global_ptr(J.cons), global_ptr(J.status), win, global_ptr(J.map));
 }
@@ -1659,7 +1725,7 @@ int run_reconstruct(RecoWork* w, const void* B, int is_f32, double2* phases, uin
                     s0, e0, win, w->map);
             hipLaunchKernelGGL(k_reco_bulk, dim3(nk), dim3(kBulkThreads), 0, s, a, w->visits, w->first_pos, s0, win,
                 w->map, flags, phases, B, is_f32, w->S, w->N, w->E, w->cnt, w->rec, w->cap, w->status);
-            hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(kFixThreads), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
+            hipLaunchKernelGGL(k_reco_fixup, dim3(1), dim3(kPushThreads), 0, s, a, w->visits, w->first_pos, s0, e0, w->S, w->N,
                 w->E, w->cnt, w->rec, w->cap, flags, phases, consistency, w->status, win, w->map);
         }
         SMIP_HIP(hipGetLastError());
@@ -1725,7 +1791,7 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
             if (win == 0) // later windows' maps are written by the fixups before them
                 hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
             hipLaunchKernelGGL(k_reco_bulk_many, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
-            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(kFixThreads), 0, s, dtab, win);
+            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(kGenThreads), 0, s, dtab, win);
         }
         SMIP_HIP(hipGetLastError());
         std::vector<int32_t> st(4 * (size_t)n);

@@ -57,7 +57,7 @@ def test_finalize_retry_after_failed_recovery(ctx, monkeypatch, grouped):
     p = S.job_params(64, 64, 16, 24, np.uint8, frames_per_batch=4)
     ref = S.Job(ctx, p)
     ref.push_frames_host(frames)
-    ref.finalize()
+    S.finalize_many([ref]) if grouped else ref.finalize()
     want = ref.result()
     ref.close()
 

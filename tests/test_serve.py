@@ -371,7 +371,10 @@ def test_captures_concurrent_with_allocation_and_copies(ctx):
 
 def test_finalize_many_matches_single(ctx):
     """Grouped phase recovery (one launch per stage and window index for all jobs)
-    gives each job exactly what its own finalize gives."""
+    gives each job exactly what a group of that job alone gives, and what its own
+    finalize gives up to the order of the FP64 phase sums: grouped recoveries run
+    the general windowed schedule, a single one the push chain where a window
+    allows it.  Flags and every input of the recursion are identical."""
     spec = [("binary128", 64, 30, 16, 32), ("triple256", 128, 20, 32, 64), ("extended512", 64, 25, 20, 40),
             ("binary128", 32, 40, 12, 16), ("triple256", 128, 18, 24, 24)]
     data = [(_frames(kind, N, n, seed_offset=40 + i), S.job_params(N, N, b, p, np.dtype(
@@ -391,8 +394,19 @@ def test_finalize_many_matches_single(ctx):
         job.push_frames_host(fr)
         jobs.append(job)
     S.finalize_many(jobs)
-    for k, job in enumerate(jobs):
-        _assert_identical(_outputs(job), single[k])
+    for k, (job, (fr, prm)) in enumerate(zip(jobs, data)):
+        got = _outputs(job)
+        alone = S.Job(ctx, prm)
+        alone.push_frames_host(fr)
+        S.finalize_many([alone])
+        _assert_identical(got, _outputs(alone))
+        alone.close()
+        for key in ("bispectrum", "powerspec", "sum_image", "shifts", "flags"):
+            assert np.array_equal(got[key], single[k][key]), key
+        dphi = float(np.max(np.abs(got["phases"] - single[k]["phases"])))
+        dimg = float(np.linalg.norm(got["image"] - single[k]["image"]) / np.linalg.norm(single[k]["image"]))
+        print(f"job {k}: grouped vs single recovery: max |dphase| {dphi:.2e}, image rel L2 {dimg:.2e}")
+        assert dphi <= 1e-9 and dimg <= 1e-8
         assert job.stats()["reco_calls"] == calls[k]
         job.close()
 
@@ -422,7 +436,7 @@ def test_add_frames_many_matches_single_jobs(ctx):
     uneven frame side, frame count, depth, sample type and tile width -- plus an
     FP64 job and a 96x64 job, which take their own launches -- gives each job
     exactly what smip_job_add_frames gives it, also when the frames arrive over
-    several grouped calls."""
+    several grouped calls (both sides recover their phases in one group)."""
     specs = [("binary128", 64, 150, 16, 0, S.PRECISION_MIXED), ("triple256", 128, 90, 24, 9, S.PRECISION_MIXED),
              ("extended512", 64, 40, 33, 0, S.PRECISION_MIXED), ("triple256", 256, 70, 48, 25, S.PRECISION_MIXED),
              ("binary128", 32, 13, 12, 17, S.PRECISION_MIXED), ("extended512", 128, 30, 20, 13, S.PRECISION_MIXED),
@@ -451,7 +465,7 @@ def test_add_frames_many_matches_single_jobs(ctx):
             for piece in split:
                 if piece[k][1]:
                     j.add_frames(st, piece[k][0], piece[k][1])
-            j.finalize()
+            S.finalize_many([j])
             want.append(_outputs(j))
             j.close()
         jobs = [S.Job(ctx, p) for p in params]
