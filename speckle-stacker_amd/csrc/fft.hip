@@ -281,6 +281,29 @@ __device__ inline int dif_pos(int k, int log2n)
     return half | (int)r;
 }
 
+// Twiddle tables of at most 4 KiB are copied into LDS behind a kernel's tile, so
+// that each FFT stage reads its twiddles beside its data instead of waiting on a
+// memory round trip (5-6% on the 256- and 512-point FP32 passes and the 256-point
+// FP64 ones).  Larger tables stay in memory: 8 KiB more per workgroup took the
+// 512-point FP64 column pass from four workgroups per CU to three, and made it
+// slower.  tw_lds_bytes is the LDS a launch adds for a transform of length n.
+template <typename V>
+__host__ __device__ constexpr bool tw_staged(int n) { return sizeof(V) * (size_t)n <= 4096; }
+template <typename V>
+__host__ __device__ constexpr size_t tw_lds_bytes(int n) { return tw_staged<V>(n) ? sizeof(V) * (size_t)n : 0; }
+// Copies tw[0, n) to `at` when tw_staged(n) and returns the table to use; the
+// caller's next barrier publishes it.
+template <typename V>
+__device__ inline const V* stage_twiddles(const V* __restrict__ tw, int n, char* at)
+{
+    if (!tw_staged<V>(n))
+        return tw;
+    V* T = reinterpret_cast<V*>(at);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        T[i] = tw[i];
+    return T;
+}
+
 template <typename S, typename In>
 __device__ inline S load_sample(const In* p, int64_t i, bool absval)
 {
@@ -345,8 +368,9 @@ __device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int
             return mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
         },
         [&](int t, const V& v) { A[t] = v; });
+    const V* T = stage_twiddles<V>(tw, X, smem + 2 * sizeof(V) * ppw * X);
     __syncthreads();
-    const V* R = lds_fft<V>(A, B, X, lx, ppw, tw, false);
+    const V* R = lds_fft<V>(A, B, X, lx, ppw, T, false);
     const S half = (S)0.5;
     // t / hx by a multiply with ceil(2^32 / hx): exact for t < 2^32 / hx
     const uint32_t hx_inv = 0xffffffffu / (uint32_t)hx + 1u;
@@ -413,8 +437,9 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
             const int y = t >> lcb, c = t & (cb - 1);
             A[c * ld + col_at<V>(y)] = v;
         });
+    const V* T = stage_twiddles<V>(tw, Y, smem + sizeof(V) * cb * ld);
     __syncthreads();
-    lds_fft_dif<V>(A, Y, ly, cb, ld, tw, false);
+    lds_fft_dif<V>(A, Y, ly, cb, ld, T, false);
     const V* R = A; // digit-reversed: X[k] at dif_pos(k)
     if (MODE == 0) {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -560,8 +585,9 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
             const int y = t >> lcb, c = t & (cb - 1);
             A[c * ld + col_at<double2>(y)] = v;
         });
+    const double2* T = stage_twiddles<double2>(tw, Y, smem + sizeof(double2) * cb * ld);
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, false);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, T, false);
     // back to natural order through registers, multiplied on the way
     double2 v[kPer];
 #pragma unroll
@@ -586,7 +612,7 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
             A[(t >> ly) * ld + col_at<double2>(t & (Y - 1))] = v[q];
     }
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, true);
+    lds_fft_dif<double2>(A, Y, ly, cb, ld, T, true);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
@@ -693,8 +719,9 @@ __device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__
             return cadd(h0, cmul_pi(h1));
         },
         [&](int t, const double2& v) { A[t] = v; });
+    const double2* T = stage_twiddles<double2>(tw, X, smem + 2 * sizeof(double2) * ppw * X);
     __syncthreads();
-    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
+    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, T, true);
     // the scan starts from element 0 (max_element's first candidate) in the
     // workgroup that holds it, from nothing elsewhere
     double bv = -INFINITY;
@@ -1017,7 +1044,7 @@ static void launch_rows_fwd(const FftPlan* p, const void* frames, int dtype, int
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int ppw = rows_per_wg<V>(X, Y / 2);
     dim3 grid((Y / 2) / ppw, (unsigned)nf);
-    const size_t lds = 2 * sizeof(V) * ppw * X;
+    const size_t lds = 2 * sizeof(V) * ppw * X + tw_lds_bytes<V>(X);
     switch (dtype) {
     case SMIP_DTYPE_U8:
         hipLaunchKernelGGL((k_rows_fwd<V, uint8_t>), grid, dim3(kThreads), lds, s, (const uint8_t*)frames, X, Y, p->lx,
@@ -1047,7 +1074,7 @@ static void launch_cols_fwd(const FftPlan* p, const V* G, int64_t nf, Out* out, 
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int cb = cols_per_wg<V>(Y);
     dim3 grid((hx + cb - 1) / cb, (unsigned)nf);
-    const size_t lds = sizeof(V) * cb * col_ld<V>(Y);
+    const size_t lds = sizeof(V) * cb * col_ld<V>(Y) + tw_lds_bytes<V>(Y);
     hipLaunchKernelGGL((k_cols_fwd<V, MODE, Out>), grid, dim3(kThreads), lds, s, G, X, Y, p->ly, hx, cb, tw_y<V>(p),
         out, hp);
     SMIP_HIP(hipGetLastError());
@@ -1110,12 +1137,13 @@ void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t n
     launch_rows_fwd<double2>(p, frames, dtype, nf, absval, work, s);
     const int cb = cols_per_wg<double2>(Y); // cb * Y * 16 <= kColLds, as k_xcorr_cols's registers assume
     hipLaunchKernelGGL(k_xcorr_cols, dim3((hx + cb - 1) / cb, (unsigned)nf), dim3(kThreads),
-        sizeof(double2) * cb * col_ld<double2>(Y), s, ref_spec, X, Y, p->ly, hx, cb, p->twy64, work);
+        sizeof(double2) * cb * col_ld<double2>(Y) + tw_lds_bytes<double2>(Y), s, ref_spec, X, Y, p->ly, hx, cb,
+        p->twy64, work);
     SMIP_HIP(hipGetLastError());
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;
-    hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads), 2 * sizeof(double2) * ppw * X,
-        s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
+    hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads),
+        2 * sizeof(double2) * ppw * X + tw_lds_bytes<double2>(X), s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
     SMIP_HIP(hipGetLastError());
     launch_best_to_shift(best, nblk, nf, X, Y, shifts, s);
 }
@@ -1150,10 +1178,10 @@ FftTiling fft_tiling(const FftPlan* p)
     t.ppw64 = rows_per_wg<double2>(X, Y / 2);
     t.cb64 = cols_per_wg<double2>(Y);
     t.nblk64 = (Y / 2) / t.ppw64;
-    t.lds_rows32 = 2 * sizeof(float2) * t.ppw32 * X;
-    t.lds_cols32 = sizeof(float2) * t.cb32 * col_ld<float2>(Y);
-    t.lds_rows64 = 2 * sizeof(double2) * t.ppw64 * X;
-    t.lds_cols64 = sizeof(double2) * t.cb64 * col_ld<double2>(Y);
+    t.lds_rows32 = 2 * sizeof(float2) * t.ppw32 * X + tw_lds_bytes<float2>(X);
+    t.lds_cols32 = sizeof(float2) * t.cb32 * col_ld<float2>(Y) + tw_lds_bytes<float2>(Y);
+    t.lds_rows64 = 2 * sizeof(double2) * t.ppw64 * X + tw_lds_bytes<double2>(X);
+    t.lds_cols64 = sizeof(double2) * t.cb64 * col_ld<double2>(Y) + tw_lds_bytes<double2>(Y);
     return t;
 }
 

@@ -1319,7 +1319,7 @@ __global__ void k_reco_map(const int2* __restrict__ visits, const int32_t* __res
     reco_map_body(blockIdx.x, visits, first_pos, a, s, e, win, map);
 }
 
-__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits,
+__global__ __launch_bounds__(kBulkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_reco_bulk(RecoArgs a, const int2* __restrict__ visits,
     const int32_t* __restrict__ first_pos, int64_t s, int64_t win, const int64_t* __restrict__ map,
     const uint8_t* __restrict__ flags, const double2* __restrict__ phases, const void* __restrict__ B, int is_f32,
     double2* __restrict__ S, int32_t* __restrict__ N, int32_t* __restrict__ E, int32_t* __restrict__ cnt,
@@ -1369,7 +1369,7 @@ __global__ void k_reco_map_many(const RecoJob* __restrict__ jobs, int64_t win)
         global_ptr(J.map));
 }
 
-__global__ __launch_bounds__(kBulkThreads) void k_reco_bulk_many(const RecoJob* __restrict__ jobs, int64_t win)
+__global__ __launch_bounds__(kBulkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_reco_bulk_many(const RecoJob* __restrict__ jobs, int64_t win)
 {
     const RecoJob& J = jobs[blockIdx.y];
     const RecoArgs a = J.a;
