@@ -2651,8 +2651,10 @@ int smip_amd::job_finalize_many_on(smip_job* const* jobs, uint64_t n, int32_t* s
             hipEvent_t t0 = event(true), t1 = event(true);
             injected_failure("reco");
             SMIP_HIP(hipEventRecord(t0, cs));
+            // the server's recoveries (on its own-queue streams) keep the general
+            // schedule: the one hang seen with the push chain was there
             run_reconstruct_many(work.data(), (int)n, B.data(), f32.data(), ph.data(), fl.data(), co.data(),
-                calls.data(), st.data(), cs);
+                calls.data(), st.data(), cs, group_stream == nullptr);
             SMIP_HIP(hipEventRecord(t1, cs));
             SMIP_HIP(hipEventSynchronize(t1));
             float reco_ms = 0.f;

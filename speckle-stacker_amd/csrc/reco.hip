@@ -1369,30 +1369,32 @@ __global__ void k_reco_map_many(const RecoJob* __restrict__ jobs, int64_t win)
         global_ptr(J.map));
 }
 
-__global__ __launch_bounds__(kBulkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_reco_bulk_many(const RecoJob* __restrict__ jobs, int64_t win)
+template <bool PUSH>
+__global__ __launch_bounds__(kBulkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_reco_bulk_many(
+    const RecoJob* __restrict__ jobs, int64_t win)
 {
     const RecoJob& J = jobs[blockIdx.y];
     const RecoArgs a = J.a;
     const int64_t s0 = win * kWindow;
     if (s0 + (int64_t)blockIdx.x >= a.nvisits)
         return;
-    reco_bulk_body<false>(blockIdx.x, a, global_ptr(J.visits), global_ptr(J.first_pos), s0, win, global_ptr(J.map),
-        global_ptr(J.flags),
-        global_ptr(J.phases), global_ptr(J.B), J.is_f32, global_ptr(J.S), global_ptr(J.N), global_ptr(J.E),
-        global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
+    reco_bulk_body<PUSH>(blockIdx.x, a, global_ptr(J.visits), global_ptr(J.first_pos), s0, win, global_ptr(J.map),
+        global_ptr(J.flags), global_ptr(J.phases), global_ptr(J.B), J.is_f32, global_ptr(J.S), global_ptr(J.N),
+        global_ptr(J.E), global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.status));
 }
 
-__global__ __launch_bounds__(kGenThreads) void k_reco_fixup_many(const RecoJob* __restrict__ jobs, int64_t win)
+template <bool PUSH>
+__global__ __launch_bounds__(PUSH ? kPushThreads : kGenThreads) void k_reco_fixup_many(
+    const RecoJob* __restrict__ jobs, int64_t win)
 {
     const RecoJob& J = jobs[blockIdx.x];
     const RecoArgs a = J.a;
     const int64_t s0 = win * kWindow;
     if (s0 >= a.nvisits)
         return;
-    reco_fixup_body<false>(a, global_ptr(J.visits), global_ptr(J.first_pos), s0, min(s0 + kWindow, a.nvisits),
-        global_ptr(J.S),
-        global_ptr(J.N), global_ptr(J.E), global_ptr(J.cnt), global_ptr(J.rec), J.cap, global_ptr(J.flags),
-        global_ptr(J.phases), global_ptr(J.cons), global_ptr(J.status), win, global_ptr(J.map));
+    reco_fixup_body<PUSH>(a, global_ptr(J.visits), global_ptr(J.first_pos), s0, min(s0 + kWindow, a.nvisits),
+        global_ptr(J.S), global_ptr(J.N), global_ptr(J.E), global_ptr(J.cnt), global_ptr(J.rec), J.cap,
+        global_ptr(J.flags), global_ptr(J.phases), global_ptr(J.cons), global_ptr(J.status), win, global_ptr(J.map));
 }
 
 // ---------------------------------------------------------------------------
@@ -1747,12 +1749,13 @@ const char* reco_error_text(int32_t code)
 
 // Phase recovery of n bispectra at once: per window index one map, one bulk and
 // one fixup launch cover every job that has that window, so the single-wave
-// fixups of different jobs run side by side.  Each job's arithmetic and order
-// are those of run_reconstruct.  calls[k] receives job k's calc_phase count and
-// status[k] its status (SMIP_ERR_OUT_OF_RANGE where get_element would throw).
+// fixups of different jobs run side by side.  With push, each job's arithmetic
+// and order are those of run_reconstruct; without, every window takes the
+// general schedule.  calls[k] receives job k's calc_phase count and status[k] its
+// status (SMIP_ERR_OUT_OF_RANGE where get_element would throw).
 void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const int* is_f32,
     double2* const* phases, uint8_t* const* flags, double* const* consistency, int32_t* calls, int32_t* status,
-    hipStream_t s)
+    hipStream_t s, bool push)
 {
     if (n <= 0)
         return;
@@ -1790,8 +1793,13 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
         for (int64_t win = 0; win < maxwin; ++win) {
             if (win == 0) // later windows' maps are written by the fixups before them
                 hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
-            hipLaunchKernelGGL(k_reco_bulk_many, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
-            hipLaunchKernelGGL(k_reco_fixup_many, dim3(n), dim3(kGenThreads), 0, s, dtab, win);
+            if (push) {
+                hipLaunchKernelGGL(k_reco_bulk_many<true>, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_fixup_many<true>, dim3(n), dim3(kPushThreads), 0, s, dtab, win);
+            } else {
+                hipLaunchKernelGGL(k_reco_bulk_many<false>, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_fixup_many<false>, dim3(n), dim3(kGenThreads), 0, s, dtab, win);
+            }
         }
         SMIP_HIP(hipGetLastError());
         std::vector<int32_t> st(4 * (size_t)n);

@@ -412,10 +412,11 @@ RecoWork* reco_work_create(const Desc& d, std::shared_ptr<const RecoGeom> geom);
 uint64_t reco_scratch_bytes(const Desc& d, int32_t xsize, int32_t ysize);
 void reco_work_destroy(RecoWork*);
 // n recoveries together (one launch per stage and window index); calls[k] = job k's
-// calc_phase count, status[k] = its status
+// calc_phase count, status[k] = its status.  push: the push chain where a window
+// allows it (as a single recovery), else the general schedule throughout.
 void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const int* is_f32,
     double2* const* phases, uint8_t* const* flags, double* const* consistency, int32_t* calls, int32_t* status,
-    hipStream_t s);
+    hipStream_t s, bool push);
 void run_calc_phase(const Desc& d, int32_t xsize, int32_t ysize, const void* B, int is_f32, double2* phases,
     uint8_t* flags, double* consistency, int32_t wx, int32_t wy, hipStream_t s);
 // message for a nonzero recovery status code
