@@ -51,8 +51,9 @@ std::recursive_mutex& capture_guard()
 // thread queued behind the allocation lock).  Blocks that their owner has shown
 // to be idle (all streams that used them synchronised: IdleRelease scope) are
 // kept here instead, per device, and handed out again for requests of the same
-// size up to 1/8 smaller.  Cached blocks count as free memory; an allocation that
-// fails returns the device's cached blocks to HIP and tries again.
+// size up to 1/8 smaller, up to half of the device's memory.  Cached blocks count
+// as free memory; an allocation that fails returns the device's cached blocks to
+// HIP and tries again.
 namespace {
 thread_local int g_idle_release = 0;
 
@@ -93,7 +94,7 @@ size_t device_cap(BlockCache& c, int dev)
         (void)hipGetLastError();
         total = 0;
     }
-    return c.dev_cap[dev] = total / 8;
+    return c.dev_cap[dev] = total / 2;
 }
 
 int current_device()
@@ -244,11 +245,45 @@ hipError_t mem_get_info(size_t* free_b, size_t* total_b)
     return e;
 }
 
+// Stream pool.  hipStreamDestroy takes 0.6-2 ms, longer while the device is busy,
+// and every job makes and releases up to four streams, so released ordinary streams
+// (idle: synchronised, not capturing) are kept per device and handed out again.
+// Streams with a CU mask (a hardware queue of their own) are destroyed.  Like the
+// block cache, the pool is never destroyed: nothing touches HIP at process exit.
+namespace {
+struct StreamPool {
+    std::mutex mu;
+    std::unordered_map<hipStream_t, int> ordinary; // made by make_stream: its device
+    std::map<int, std::vector<hipStream_t>> idle;  // per device
+    static constexpr size_t kMaxIdle = 256;
+    static StreamPool& get()
+    {
+        static auto* p = new StreamPool;
+        return *p;
+    }
+};
+} // namespace
+
 hipStream_t make_stream()
 {
-    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    const int dev = current_device();
+    StreamPool& P = StreamPool::get();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& v = P.idle[dev];
+        if (!v.empty()) {
+            hipStream_t s = v.back();
+            v.pop_back();
+            return s;
+        }
+    }
     hipStream_t s = nullptr;
-    SMIP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        SMIP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    }
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.ordinary[s] = dev;
     return s;
 }
 
@@ -280,7 +315,22 @@ void destroy_stream(hipStream_t s)
 {
     if (!s)
         return;
-    (void)hipStreamSynchronize(s);
+    const bool idle = hipStreamSynchronize(s) == hipSuccess;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool plain = idle && hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+    StreamPool& P = StreamPool::get();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.ordinary.find(s);
+        if (it != P.ordinary.end()) {
+            auto& v = P.idle[it->second];
+            if (plain && v.size() < StreamPool::kMaxIdle) {
+                v.push_back(s);
+                return;
+            }
+            P.ordinary.erase(it);
+        }
+    }
     std::lock_guard<std::recursive_mutex> lk(capture_guard());
     (void)hipStreamDestroy(s);
 }
