@@ -325,6 +325,155 @@ constexpr int kColPer = kColLds / (int)sizeof(V) / kThreads;
 constexpr int kRowElems = 2048;
 constexpr int kRowPer = kRowElems / kThreads;
 
+// Columns per workgroup of a column pass (cols_per_wg on the host): as many as
+// kColLds holds, at most 64, a power of two; 0 when one column does not fit.
+template <typename V>
+__host__ __device__ constexpr int col_count(int log2n)
+{
+    const long k0 = (long)kColLds / ((long)sizeof(V) << log2n);
+    const int k = (int)(k0 > 64 ? 64 : k0);
+    int p = 1;
+    while (p * 2 <= k)
+        p *= 2;
+    return k < 1 ? 0 : p;
+}
+
+// lds_fft_dif with the length, the column count (col_count) and the direction
+// fixed at compile time: the same butterflies with the same arithmetic in the
+// same order, so the results are identical, but with the index arithmetic
+// folded, the loads of all of a thread's butterflies of a stage issued before
+// any of them is computed, and the twiddle table's address space known (LDS
+// when it is staged, global memory otherwise) instead of generic loads that
+// wait on both counters one butterfly at a time.
+template <typename V, int LOG2N, bool INV, int LL>
+__device__ __forceinline__ void dif_radix4_ct(V* a, const V* T)
+{
+    if constexpr (LL >= 2) {
+        constexpr int N = 1 << LOG2N, NB = col_count<V>(LOG2N), LD = col_ld<V>(N);
+        constexpr int Q = 1 << (LL - 2), L = 1 << LL, STEP = N >> LL;
+        constexpr int TOT = NB * (N >> 2), IT = (TOT + kThreads - 1) / kThreads;
+        constexpr bool FULL = TOT % kThreads == 0;
+        const int tid = threadIdx.x;
+        V x[IT][4], w[IT][3];
+        int at0[IT], jj[IT];
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> (LOG2N - 2), b = t & ((N >> 2) - 1);
+                const int g = b >> (LL - 2), j = b & (Q - 1);
+                jj[q] = j;
+                at0[q] = f * LD;
+                const int i0 = g * L + j;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    x[q][r] = a[at0[q] + col_at<V>(i0 + r * Q)];
+                if constexpr (Q > 1) {
+                    w[q][0] = T[STEP * j];
+                    w[q][1] = T[2 * STEP * j];
+                    w[q][2] = T[3 * STEP * j];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int b = t & ((N >> 2) - 1);
+                const int g = b >> (LL - 2), j = jj[q];
+                const int i0 = g * L + j;
+                const V x0 = x[q][0], x1 = x[q][1], x2 = x[q][2], x3 = x[q][3];
+                const V a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+                const V a3 = INV ? cmul_pi(csub(x1, x3)) : cmul_mi(csub(x1, x3));
+                V y1 = cadd(a1, a3), y2 = csub(a0, a2), y3 = csub(a1, a3);
+                if constexpr (Q > 1) {
+                    if (j > 0) {
+                        const V w1 = INV ? cconj(w[q][0]) : w[q][0];
+                        const V w2 = INV ? cconj(w[q][1]) : w[q][1];
+                        const V w3 = INV ? cconj(w[q][2]) : w[q][2];
+                        y1 = cmul(y1, w1);
+                        y2 = cmul(y2, w2);
+                        y3 = cmul(y3, w3);
+                    }
+                }
+                a[at0[q] + col_at<V>(i0)] = cadd(a0, a2);
+                a[at0[q] + col_at<V>(i0 + Q)] = y1;
+                a[at0[q] + col_at<V>(i0 + 2 * Q)] = y2;
+                a[at0[q] + col_at<V>(i0 + 3 * Q)] = y3;
+            }
+        }
+        __syncthreads();
+        dif_radix4_ct<V, LOG2N, INV, LL - 2>(a, T);
+    }
+}
+
+template <typename V, int LOG2N, bool INV>
+__device__ __forceinline__ void lds_fft_dif_ct(V* a, const V* T)
+{
+    if constexpr (LOG2N & 1) {
+        constexpr int N = 1 << LOG2N, NB = col_count<V>(LOG2N), LD = col_ld<V>(N);
+        constexpr int H = N >> 1, TOT = NB * H, IT = (TOT + kThreads - 1) / kThreads;
+        constexpr bool FULL = TOT % kThreads == 0;
+        const int tid = threadIdx.x;
+        V x0[IT], x1[IT], w[IT];
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> (LOG2N - 1), j = t & (H - 1);
+                x0[q] = a[f * LD + col_at<V>(j)];
+                x1[q] = a[f * LD + col_at<V>(j + H)];
+                w[q] = T[j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> (LOG2N - 1), j = t & (H - 1);
+                const V ww = INV ? cconj(w[q]) : w[q];
+                a[f * LD + col_at<V>(j)] = cadd(x0[q], x1[q]);
+                a[f * LD + col_at<V>(j + H)] = cmul(csub(x0[q], x1[q]), ww);
+            }
+        }
+        __syncthreads();
+    }
+    dif_radix4_ct<V, LOG2N, INV, LOG2N - (LOG2N & 1)>(a, T);
+}
+
+// lds_fft_dif of a column tile of col_count(log2n) columns, through the
+// compile-time form for lengths 32..2048.  With STAGED, the twiddles sit at `at`
+// in LDS when tw_staged(n) (stage_twiddles put them there), else they are read
+// from tw.
+template <typename V, bool INV, bool STAGED>
+__device__ __forceinline__ void lds_fft_cols(V* a, int n, int log2n, int nb, int ld, const V* __restrict__ tw,
+    const char* at)
+{
+#define SMIP_DIF_CASE(LN)                                                                                             \
+    case LN:                                                                                                          \
+        if constexpr (col_count<V>(LN) > 0) {                                                                        \
+            lds_fft_dif_ct<V, LN, INV>(                                                                               \
+                a, STAGED && tw_staged<V>(1 << LN) ? reinterpret_cast<const V*>(at) : tw);                           \
+            return;                                                                                                   \
+        }                                                                                                             \
+        break;
+    if (nb == col_count<V>(log2n) && ld == col_ld<V>(n)) {
+        switch (log2n) {
+            SMIP_DIF_CASE(5)
+            SMIP_DIF_CASE(6)
+            SMIP_DIF_CASE(7)
+            SMIP_DIF_CASE(8)
+            SMIP_DIF_CASE(9)
+            SMIP_DIF_CASE(10)
+            SMIP_DIF_CASE(11)
+        default:
+            break;
+        }
+    }
+#undef SMIP_DIF_CASE
+    lds_fft_dif<V>(a, n, log2n, nb, ld, STAGED && tw_staged<V>(n) ? reinterpret_cast<const V*>(at) : tw, INV);
+}
+
 // A workgroup's tile load, issued all at once.  A loop that stores each element
 // to LDS as it arrives waits one full memory round trip per iteration; loading
 // up to KPER elements per thread into registers first keeps them all in flight.
@@ -437,9 +586,10 @@ __device__ __forceinline__ void cols_fwd_body(const V* __restrict__ G, int X, in
             const int y = t >> lcb, c = t & (cb - 1);
             A[c * ld + col_at<V>(y)] = v;
         });
-    const V* T = stage_twiddles<V>(tw, Y, smem + sizeof(V) * cb * ld);
+    char* at = smem + sizeof(V) * cb * ld;
+    (void)stage_twiddles<V>(tw, Y, at);
     __syncthreads();
-    lds_fft_dif<V>(A, Y, ly, cb, ld, T, false);
+    lds_fft_cols<V, false, true>(A, Y, ly, cb, ld, tw, at);
     const V* R = A; // digit-reversed: X[k] at dif_pos(k)
     if (MODE == 0) {
         for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
@@ -552,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols_inv(const double2* __re
             A[c * ld + col_at<double2>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, true);
+    lds_fft_cols<double2, true, false>(A, Y, ly, cb, ld, tw, nullptr);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
@@ -585,9 +735,10 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
             const int y = t >> lcb, c = t & (cb - 1);
             A[c * ld + col_at<double2>(y)] = v;
         });
-    const double2* T = stage_twiddles<double2>(tw, Y, smem + sizeof(double2) * cb * ld);
+    char* at = smem + sizeof(double2) * cb * ld;
+    (void)stage_twiddles<double2>(tw, Y, at);
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, T, false);
+    lds_fft_cols<double2, false, true>(A, Y, ly, cb, ld, tw, at);
     // back to natural order through registers, multiplied on the way
     double2 v[kPer];
 #pragma unroll
@@ -612,7 +763,7 @@ __device__ __forceinline__ void xcorr_cols_body(const double2* __restrict__ Rref
             A[(t >> ly) * ld + col_at<double2>(t & (Y - 1))] = v[q];
     }
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, T, true);
+    lds_fft_cols<double2, true, true>(A, Y, ly, cb, ld, tw, at);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         const int kx = kx0 + c;
@@ -982,7 +1133,10 @@ __global__ __launch_bounds__(kThreads) void k_cols_c2c(double2* __restrict__ dat
             A[c * ld + col_at<double2>(y)] = v;
         });
     __syncthreads();
-    lds_fft_dif<double2>(A, Y, ly, cb, ld, tw, inv != 0);
+    if (inv)
+        lds_fft_cols<double2, true, false>(A, Y, ly, cb, ld, tw, nullptr);
+    else
+        lds_fft_cols<double2, false, false>(A, Y, ly, cb, ld, tw, nullptr);
     for (int t = threadIdx.x; t < cb * Y; t += blockDim.x) {
         const int y = t >> lcb, c = t & (cb - 1);
         if (x0 + c < X)
