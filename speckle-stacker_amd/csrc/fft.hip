@@ -474,6 +474,139 @@ __device__ __forceinline__ void lds_fft_cols(V* a, int n, int log2n, int nb, int
     lds_fft_dif<V>(a, n, log2n, nb, ld, STAGED && tw_staged<V>(n) ? reinterpret_cast<const V*>(at) : tw, INV);
 }
 
+// Transforms per workgroup of a row pass when the frame has rows to spare
+// (rows_per_wg on the host): enough that a radix-4 stage has a butterfly for
+// each of the 256 lanes.
+__host__ __device__ constexpr int row_fill(int log2n)
+{
+    return (4 * kThreads) >> log2n > 1 ? (4 * kThreads) >> log2n : 1;
+}
+
+// lds_fft (Stockham, ping-pong) with the length, the transform count and the
+// direction fixed at compile time, as lds_fft_dif_ct is for the column form:
+// identical arithmetic, folded indices, loads of a stage issued together.
+// Stage S is the radix-4 stage of span Ns = 4^S; after the radix-4 stages, a
+// radix-2 stage when log2(n) is odd.  Returns the buffer holding the result.
+template <typename V, int LOG2N, int NB, bool INV, int S>
+__device__ __forceinline__ V* stockham_ct(V* a, V* b, const V* T)
+{
+    constexpr int N = 1 << LOG2N;
+    const int tid = threadIdx.x;
+    if constexpr (2 * S + 2 <= LOG2N) {
+        constexpr int Ns = 1 << (2 * S), Q4 = N >> 2, LQ4 = LOG2N - 2, LTW = LOG2N - 2 - 2 * S;
+        constexpr int TOT = NB * Q4, IT = (TOT + kThreads - 1) / kThreads;
+        constexpr bool FULL = TOT % kThreads == 0;
+        V v[IT][4], w[IT][3];
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> LQ4, j = t & (Q4 - 1);
+                const V* src = a + f * N;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    v[q][r] = src[j + r * Q4];
+                if constexpr (Ns > 1) {
+                    const int e = (j & (Ns - 1)) << LTW;
+                    w[q][0] = T[e];
+                    w[q][1] = T[2 * e];
+                    w[q][2] = T[3 * e];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> LQ4, j = t & (Q4 - 1);
+                const int k = j & (Ns - 1);
+                V v0 = v[q][0], v1 = v[q][1], v2 = v[q][2], v3 = v[q][3];
+                if constexpr (Ns > 1) {
+                    const V w1 = INV ? cconj(w[q][0]) : w[q][0];
+                    const V w2 = INV ? cconj(w[q][1]) : w[q][1];
+                    const V w3 = INV ? cconj(w[q][2]) : w[q][2];
+                    v1 = cmul(v1, w1);
+                    v2 = cmul(v2, w2);
+                    v3 = cmul(v3, w3);
+                }
+                V a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+                a3 = INV ? cmul_pi(a3) : cmul_mi(a3);
+                V* dst = b + f * N;
+                const int d = ((j - k) << 2) + k;
+                dst[d] = cadd(a0, a2);
+                dst[d + Ns] = cadd(a1, a3);
+                dst[d + 2 * Ns] = csub(a0, a2);
+                dst[d + 3 * Ns] = csub(a1, a3);
+            }
+        }
+        __syncthreads();
+        return stockham_ct<V, LOG2N, NB, INV, S + 1>(b, a, T);
+    } else if constexpr (LOG2N & 1) {
+        constexpr int Ns = 1 << (2 * S), Q2 = N >> 1, LQ2 = LOG2N - 1;
+        constexpr int TOT = NB * Q2, IT = (TOT + kThreads - 1) / kThreads;
+        constexpr bool FULL = TOT % kThreads == 0;
+        V v0[IT], v1[IT], w[IT];
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> LQ2, j = t & (Q2 - 1);
+                v0[q] = a[f * N + j];
+                v1[q] = a[f * N + j + Q2];
+                if constexpr (Ns > 1)
+                    w[q] = T[j & (Ns - 1)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            const int t = tid + q * kThreads;
+            if (FULL || t < TOT) {
+                const int f = t >> LQ2, j = t & (Q2 - 1);
+                const int k = j & (Ns - 1);
+                V x1 = v1[q];
+                if constexpr (Ns > 1)
+                    x1 = cmul(x1, INV ? cconj(w[q]) : w[q]);
+                V* dst = b + f * N;
+                const int d = ((j - k) << 1) + k;
+                dst[d] = cadd(v0[q], x1);
+                dst[d + Ns] = csub(v0[q], x1);
+            }
+        }
+        __syncthreads();
+        return b;
+    } else {
+        return a;
+    }
+}
+
+// lds_fft through the compile-time form for lengths 32..2048 when the
+// workgroup holds row_fill(log2n) transforms.  With STAGED, the twiddles sit at
+// `at` in LDS when tw_staged(n), else they are read from tw.
+template <typename V, bool INV, bool STAGED>
+__device__ __forceinline__ V* lds_fft_rows(V* a, V* b, int n, int log2n, int nb, const V* __restrict__ tw,
+    const char* at)
+{
+#define SMIP_STOCKHAM_CASE(LN)                                                                                        \
+    case LN:                                                                                                          \
+        return stockham_ct<V, LN, row_fill(LN), INV, 0>(                                                              \
+            a, b, STAGED && tw_staged<V>(1 << LN) ? reinterpret_cast<const V*>(at) : tw);
+    if (nb == row_fill(log2n)) {
+        switch (log2n) {
+            SMIP_STOCKHAM_CASE(5)
+            SMIP_STOCKHAM_CASE(6)
+            SMIP_STOCKHAM_CASE(7)
+            SMIP_STOCKHAM_CASE(8)
+            SMIP_STOCKHAM_CASE(9)
+            SMIP_STOCKHAM_CASE(10)
+            SMIP_STOCKHAM_CASE(11)
+        default:
+            break;
+        }
+    }
+#undef SMIP_STOCKHAM_CASE
+    return lds_fft<V>(a, b, n, log2n, nb, STAGED && tw_staged<V>(n) ? reinterpret_cast<const V*>(at) : tw, INV);
+}
+
 // A workgroup's tile load, issued all at once.  A loop that stores each element
 // to LDS as it arrives waits one full memory round trip per iteration; loading
 // up to KPER elements per thread into registers first keeps them all in flight.
@@ -517,9 +650,10 @@ __device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int
             return mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
         },
         [&](int t, const V& v) { A[t] = v; });
-    const V* T = stage_twiddles<V>(tw, X, smem + 2 * sizeof(V) * ppw * X);
+    char* at = smem + 2 * sizeof(V) * ppw * X;
+    (void)stage_twiddles<V>(tw, X, at);
     __syncthreads();
-    const V* R = lds_fft<V>(A, B, X, lx, ppw, T, false);
+    const V* R = lds_fft_rows<V, false, true>(A, B, X, lx, ppw, tw, at);
     const S half = (S)0.5;
     // t / hx by a multiply with ceil(2^32 / hx): exact for t < 2^32 / hx
     const uint32_t hx_inv = 0xffffffffu / (uint32_t)hx + 1u;
@@ -816,7 +950,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict
         },
         [&](int t, const double2& v) { A[t] = v; });
     __syncthreads();
-    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, tw, true);
+    const double2* Z = lds_fft_rows<double2, true, false>(A, B, X, lx, ppw, tw, nullptr);
     for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
         const int p = t >> lx, x = t & (X - 1);
         const int64_t y0 = 2 * (pair0 + p);
@@ -870,9 +1004,10 @@ __device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__
             return cadd(h0, cmul_pi(h1));
         },
         [&](int t, const double2& v) { A[t] = v; });
-    const double2* T = stage_twiddles<double2>(tw, X, smem + 2 * sizeof(double2) * ppw * X);
+    char* at = smem + 2 * sizeof(double2) * ppw * X;
+    (void)stage_twiddles<double2>(tw, X, at);
     __syncthreads();
-    const double2* Z = lds_fft<double2>(A, B, X, lx, ppw, T, true);
+    const double2* Z = lds_fft_rows<double2, true, true>(A, B, X, lx, ppw, tw, at);
     // the scan starts from element 0 (max_element's first candidate) in the
     // workgroup that holds it, from nothing elsewhere
     double bv = -INFINITY;
@@ -1109,7 +1244,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2c(double2* __restrict__ dat
     gather_tile<kRowPer, double2>(
         rpw * X, [&](int t) { return base[t]; }, [&](int t, const double2& v) { A[t] = v; });
     __syncthreads();
-    const double2* Z = lds_fft<double2>(A, B, X, lx, rpw, tw, inv != 0);
+    const double2* Z = inv ? lds_fft_rows<double2, true, false>(A, B, X, lx, rpw, tw, nullptr)
+                           : lds_fft_rows<double2, false, false>(A, B, X, lx, rpw, tw, nullptr);
     for (int t = threadIdx.x; t < rpw * X; t += blockDim.x)
         base[t] = Z[t];
 }
