@@ -305,9 +305,9 @@ __device__ inline const V* stage_twiddles(const V* __restrict__ tw, int n, char*
 }
 
 template <typename S, typename In>
-__device__ inline S load_sample(const In* p, int64_t i, bool absval)
+__device__ inline S sample_value(In x, bool absval)
 {
-    S v = (S)p[i];
+    S v = (S)x;
     return absval ? (v < (S)0 ? -v : v) : v;
 }
 
@@ -629,6 +629,40 @@ __device__ inline void gather_tile(int total, Load&& load, Store&& store)
     }
 }
 
+// A tile load whose elements need arithmetic on the loaded values: fetch(t)
+// only loads (raw values into registers), make(t, raw) converts them and stores
+// to LDS.  In gather_tile, arithmetic inside the guarded load makes each
+// element's loads wait there, one element at a time; here all of a round's loads
+// are in flight together.  Rounds of KR elements per thread bound the registers;
+// rounds past the tile are skipped (a uniform branch).
+template <int KPER, int KR, typename R, typename Fetch, typename Make>
+__device__ inline void gather_tile2(int total, Fetch&& fetch, Make&& make)
+{
+#pragma unroll
+    for (int q0 = 0; q0 < KPER; q0 += KR) {
+        if (q0 * kThreads >= total)
+            break;
+        R r[KR];
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+            const int t = (int)threadIdx.x + (q0 + q) * kThreads;
+            if (t < total)
+                r[q] = fetch(t);
+        }
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+            const int t = (int)threadIdx.x + (q0 + q) * kThreads;
+            if (t < total)
+                make(t, r[q]);
+        }
+    }
+}
+
+template <typename T>
+struct Pair {
+    T a, b;
+};
+
 // ---------------------------------------------------------------------------
 // Row pass: real rows y0 = 2p, y1 = 2p+1 of frame f -> G[f][y][kx], kx in [0, X/2]
 // ---------------------------------------------------------------------------
@@ -642,14 +676,14 @@ __device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int
     V* B = A + ppw * X;
     const int pair0 = bx * ppw;
     const In* fr = frames + f * (int64_t)X * Y;
-    gather_tile<kRowPer, V>(
+    gather_tile2<kRowPer, kRowPer, Pair<In>>(
         ppw * X,
         [&](int t) {
             const int p = t >> lx, x = t & (X - 1);
             const int64_t y0 = 2 * (pair0 + p);
-            return mk<V>(load_sample<S>(fr, y0 * X + x, absval), load_sample<S>(fr, (y0 + 1) * X + x, absval));
+            return Pair<In> { fr[y0 * X + x], fr[(y0 + 1) * X + x] };
         },
-        [&](int t, const V& v) { A[t] = v; });
+        [&](int t, const Pair<In>& r) { A[t] = mk<V>(sample_value<S>(r.a, absval), sample_value<S>(r.b, absval)); });
     char* at = smem + 2 * sizeof(V) * ppw * X;
     (void)stage_twiddles<V>(tw, X, at);
     __syncthreads();
@@ -916,6 +950,23 @@ __global__ __launch_bounds__(kThreads) void k_xcorr_cols(const double2* __restri
     xcorr_cols_body(Rref, X, Y, ly, hx, cb, tw, G, bx, f, smem);
 }
 
+// Element k of H0 + i H1 for a c2r row pair: H0, H1 are the stored half
+// spectra (kx in [0, X/2]) extended to the full length by Hermitian symmetry,
+// with the imaginary parts of the kx = 0 and kx = X/2 terms dropped.  h0, h1 are
+// the stored elements at kx = k for k <= X/2, at kx = X - k above.
+__device__ inline double2 c2r_pair_input(double2 h0, double2 h1, int k, int X)
+{
+    const bool upper = k > X / 2;
+    if (upper) {
+        h0 = cconj(h0);
+        h1 = cconj(h1);
+    } else if (k == 0 || k == X / 2) {
+        h0.y = 0.0;
+        h1.y = 0.0;
+    }
+    return cadd(h0, cmul_pi(h1));
+}
+
 // Row pass of a c2r transform, two rows at a time: z = IDFT(H0 + i H1) gives
 // row0 = Re z, row1 = Im z.  The imaginary parts of the kx = 0 and kx = X/2 terms
 // are dropped, as a c2r transform of Hermitian data does.
@@ -927,28 +978,16 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const double2* __restrict
     double2* B = A + ppw * X;
     const int64_t f = blockIdx.y;
     const int pair0 = blockIdx.x * ppw;
-    gather_tile<kRowPer, double2>(
+    gather_tile2<kRowPer, 4, Pair<double2>>(
         ppw * X,
         [&](int t) {
             const int p = t >> lx, k = t & (X - 1);
             const int64_t y0 = 2 * (pair0 + p);
             const double2* g0 = G + (f * Y + y0) * hx;
-            const double2* g1 = g0 + hx;
-            double2 h0, h1;
-            if (k <= X / 2) {
-                h0 = g0[k];
-                h1 = g1[k];
-                if (k == 0 || k == X / 2) {
-                    h0.y = 0.0;
-                    h1.y = 0.0;
-                }
-            } else {
-                h0 = cconj(g0[X - k]);
-                h1 = cconj(g1[X - k]);
-            }
-            return cadd(h0, cmul_pi(h1));
+            const int i = k > X / 2 ? X - k : k;
+            return Pair<double2> { g0[i], g0[hx + i] };
         },
-        [&](int t, const double2& v) { A[t] = v; });
+        [&](int t, const Pair<double2>& r) { A[t] = c2r_pair_input(r.a, r.b, t & (X - 1), X); });
     __syncthreads();
     const double2* Z = lds_fft_rows<double2, true, false>(A, B, X, lx, ppw, tw, nullptr);
     for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
@@ -982,28 +1021,16 @@ __device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__
     double2* A = reinterpret_cast<double2*>(smem);
     double2* B = A + ppw * X;
     const int pair0 = bx * ppw;
-    gather_tile<kRowPer, double2>(
+    gather_tile2<kRowPer, 4, Pair<double2>>(
         ppw * X,
         [&](int t) {
             const int p = t >> lx, k = t & (X - 1);
             const int64_t y0 = 2 * (pair0 + p);
             const double2* g0 = G + (f * Y + y0) * hx;
-            const double2* g1 = g0 + hx;
-            double2 h0, h1;
-            if (k <= X / 2) {
-                h0 = g0[k];
-                h1 = g1[k];
-                if (k == 0 || k == X / 2) {
-                    h0.y = 0.0;
-                    h1.y = 0.0;
-                }
-            } else {
-                h0 = cconj(g0[X - k]);
-                h1 = cconj(g1[X - k]);
-            }
-            return cadd(h0, cmul_pi(h1));
+            const int i = k > X / 2 ? X - k : k;
+            return Pair<double2> { g0[i], g0[hx + i] };
         },
-        [&](int t, const double2& v) { A[t] = v; });
+        [&](int t, const Pair<double2>& r) { A[t] = c2r_pair_input(r.a, r.b, t & (X - 1), X); });
     char* at = smem + 2 * sizeof(double2) * ppw * X;
     (void)stage_twiddles<double2>(tw, X, at);
     __syncthreads();

@@ -81,8 +81,10 @@ void launch_argmax_shift(const double* corr, int64_t nf, int32_t ncols, int32_t 
     SMIP_HIP(hipGetLastError());
 }
 
+// |frames[f]| shifted, in the sample type (0 outside the window): the absolute
+// value and the widening to double are exact in either order.
 template <typename In>
-__device__ inline double shifted_abs(const In* __restrict__ frames, int64_t f, int32_t X, int32_t Y, int32_t col,
+__device__ inline In shifted_abs_raw(const In* __restrict__ frames, int64_t f, int32_t X, int32_t Y, int32_t col,
     int32_t row, const int32_t* __restrict__ shifts)
 {
     // distance = -shift; shifted(d)(col, row) = in(col - d0, row - d1) on the
@@ -90,12 +92,14 @@ __device__ inline double shifted_abs(const In* __restrict__ frames, int64_t f, i
     const int32_t dx = -shifts[2 * f], dy = -shifts[2 * f + 1];
     const int32_t sx = min(max(dx, 0), X - 1), ex = min(max(dx + X, 1), X);
     const int32_t sy = min(max(dy, 0), Y - 1), ey = min(max(dy + Y, 1), Y);
-    double v = 0.0;
-    if (col >= sx && col < ex && row >= sy && row < ey) {
-        const double xd = (double)frames[f * (int64_t)X * Y + (int64_t)(row - dy) * X + (col - dx)];
-        v = xd < 0 ? -xd : xd;
-    }
-    return v;
+    // the load is unconditional (from a clamped, in-frame position) and masked
+    // afterwards, so that a thread's loads of many frames are all in flight
+    // together instead of each waiting inside its own branch
+    const bool in = col >= sx && col < ex && row >= sy && row < ey;
+    const int32_t r = min(max(row - dy, 0), Y - 1), c = min(max(col - dx, 0), X - 1);
+    const In x = frames[f * (int64_t)X * Y + (int64_t)r * X + c];
+    const In a = x < (In)0 ? -x : x;
+    return in ? a : (In)0;
 }
 
 template <typename In>
@@ -108,17 +112,18 @@ __device__ __forceinline__ void saa_add_body(const In* __restrict__ frames, int6
     double acc = sum[p];
     int64_t f = 0;
     // sixteen frames' loads in flight at once, added in frame order
-    for (; f + 16 <= nf; f += 16) {
-        double v[16];
+    constexpr int U = 16;
+    for (; f + U <= nf; f += U) {
+        In v[U];
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            v[k] = shifted_abs(frames, f + k, X, Y, col, row, shifts);
+        for (int k = 0; k < U; ++k)
+            v[k] = shifted_abs_raw(frames, f + k, X, Y, col, row, shifts);
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            acc += v[k];
+        for (int k = 0; k < U; ++k)
+            acc += (double)v[k];
     }
     for (; f < nf; ++f)
-        acc += shifted_abs(frames, f, X, Y, col, row, shifts);
+        acc += (double)shifted_abs_raw(frames, f, X, Y, col, row, shifts);
     sum[p] = acc;
 }
 
@@ -150,13 +155,13 @@ __device__ __forceinline__ void saa_add_int_body(const In* __restrict__ frames, 
         double v[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            v[k] = shifted_abs(frames, f + k, X, Y, col, row, shifts);
+            v[k] = (double)shifted_abs_raw(frames, f + k, X, Y, col, row, shifts);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             acc += v[k];
     }
     for (; f < f1; ++f)
-        acc += shifted_abs(frames, f, X, Y, col, row, shifts);
+        acc += (double)shifted_abs_raw(frames, f, X, Y, col, row, shifts);
     if (f1 > f0)
         atomicAdd(sum + p, acc);
 }
