@@ -266,10 +266,19 @@ __device__ void lds_fft_dif(V* a, int n, int log2n, int nb, int ld, const V* __r
     }
 }
 
-// position of X[k] after lds_fft_dif: the radix-2 split (odd log2n) selects the
-// half by k's lowest bit, then the base-4 digits of the rest reverse
+// Column transforms of 64 and 512 points run as radix-8 stages (lds_fft_dif_ct);
+// the other lengths as radix-4 stages after a radix-2 one when log2n is odd.
+__host__ __device__ constexpr bool dif_radix8(int log2n) { return log2n == 6 || log2n == 9; }
+
+// position of X[k] after the in-place DIF: for radix-8 lengths the base-8
+// digits of k reverse; otherwise the radix-2 split (odd log2n) selects the half
+// by k's lowest bit, then the base-4 digits of the rest reverse
 __device__ inline int dif_pos(int k, int log2n)
 {
+    if (dif_radix8(log2n)) {
+        const int lo = k & 7, mid = (k >> 3) & 7, hi = k >> 6; // hi = 0 for 64 points
+        return log2n == 9 ? (lo << 6) | (mid << 3) | hi : (lo << 3) | mid;
+    }
     // base-4 digit reversal of the m = log2n - odd bits: a bit reversal, then the
     // two bits of each digit swapped back
     const int odd = log2n & 1;
@@ -338,21 +347,23 @@ __host__ __device__ constexpr int col_count(int log2n)
     return k < 1 ? 0 : p;
 }
 
-// lds_fft_dif with the length, the column count (col_count) and the direction
-// fixed at compile time: the same butterflies with the same arithmetic in the
+// lds_fft_dif of nb <= NBMAX transforms (columns of a column tile, rows of a
+// row tile; EXACT: nb = NBMAX) at stride col_ld, with the length, NBMAX and the
+// direction fixed at compile time: the same butterflies with the same arithmetic in the
 // same order, so the results are identical, but with the index arithmetic
 // folded, the loads of all of a thread's butterflies of a stage issued before
 // any of them is computed, and the twiddle table's address space known (LDS
 // when it is staged, global memory otherwise) instead of generic loads that
 // wait on both counters one butterfly at a time.
-template <typename V, int LOG2N, bool INV, int LL>
-__device__ __forceinline__ void dif_radix4_ct(V* a, const V* T)
+template <typename V, int LOG2N, int NBMAX, bool EXACT, bool INV, int LL>
+__device__ __forceinline__ void dif_radix4_ct(V* a, int nb, const V* T)
 {
     if constexpr (LL >= 2) {
-        constexpr int N = 1 << LOG2N, NB = col_count<V>(LOG2N), LD = col_ld<V>(N);
+        constexpr int N = 1 << LOG2N, LD = col_ld<V>(N);
         constexpr int Q = 1 << (LL - 2), L = 1 << LL, STEP = N >> LL;
-        constexpr int TOT = NB * (N >> 2), IT = (TOT + kThreads - 1) / kThreads;
-        constexpr bool FULL = TOT % kThreads == 0;
+        constexpr int IT = (NBMAX * (N >> 2) + kThreads - 1) / kThreads;
+        constexpr bool FULL = EXACT && (NBMAX * (N >> 2)) % kThreads == 0;
+        const int TOT = (EXACT ? NBMAX : nb) * (N >> 2);
         const int tid = threadIdx.x;
         V x[IT][4], w[IT][3];
         int at0[IT], jj[IT];
@@ -403,42 +414,140 @@ __device__ __forceinline__ void dif_radix4_ct(V* a, const V* T)
             }
         }
         __syncthreads();
-        dif_radix4_ct<V, LOG2N, INV, LL - 2>(a, T);
+        dif_radix4_ct<V, LOG2N, NBMAX, EXACT, INV, LL - 2>(a, nb, T);
     }
 }
 
-template <typename V, int LOG2N, bool INV>
-__device__ __forceinline__ void lds_fft_dif_ct(V* a, const V* T)
+// 4-point DFT of (v0, v1, v2, v3) with w_4 = -i (+i when INV), in the operation
+// order of the radix-4 butterflies above.
+template <typename V, bool INV>
+__device__ __forceinline__ void dft4(const V& v0, const V& v1, const V& v2, const V& v3, V& y0, V& y1, V& y2, V& y3)
 {
-    if constexpr (LOG2N & 1) {
-        constexpr int N = 1 << LOG2N, NB = col_count<V>(LOG2N), LD = col_ld<V>(N);
-        constexpr int H = N >> 1, TOT = NB * H, IT = (TOT + kThreads - 1) / kThreads;
-        constexpr bool FULL = TOT % kThreads == 0;
+    const V a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+    const V a3 = INV ? cmul_pi(csub(v1, v3)) : cmul_mi(csub(v1, v3));
+    y0 = cadd(a0, a2);
+    y1 = cadd(a1, a3);
+    y2 = csub(a0, a2);
+    y3 = csub(a1, a3);
+}
+
+// Radix-8 stage of span L (E = L/8) of the in-place DIF, butterfly j in [0, E)
+// of group g: x_r = a[g L + j + r E]; X_s = sum_r x_r w_8^(r s), as two 4-point
+// DFTs, of u_r = x_r + x_(r+4) (even s) and of (x_r - x_(r+4)) w_8^r (odd s);
+// output s, scaled by w_L^(s j), goes to a[g L + j + s E].  Three such stages
+// make a 512-point transform where radix-4 needs five, with fewer LDS round
+// trips and barriers per point.
+template <typename V, int LOG2N, int NBMAX, bool EXACT, bool INV, int LL>
+__device__ __forceinline__ void dif_radix8_ct(V* a, int nb, const V* T)
+{
+    if constexpr (LL >= 3) {
+        using S = typename scalar_of<V>::type;
+        constexpr int N = 1 << LOG2N, LD = col_ld<V>(N);
+        constexpr int E = 1 << (LL - 3), L = 1 << LL, STEP = N >> LL;
+        constexpr int IT = (NBMAX * (N >> 3) + kThreads - 1) / kThreads;
+        constexpr bool FULL = EXACT && (NBMAX * (N >> 3)) % kThreads == 0;
+        const int TOT = (EXACT ? NBMAX : nb) * (N >> 3);
+        const S h = (S)0.70710678118654752440084436210485; // 1/sqrt(2)
         const int tid = threadIdx.x;
-        V x0[IT], x1[IT], w[IT];
+        V x[IT][8], w[IT][7];
+        int jj[IT];
 #pragma unroll
         for (int q = 0; q < IT; ++q) {
             const int t = tid + q * kThreads;
             if (FULL || t < TOT) {
-                const int f = t >> (LOG2N - 1), j = t & (H - 1);
-                x0[q] = a[f * LD + col_at<V>(j)];
-                x1[q] = a[f * LD + col_at<V>(j + H)];
-                w[q] = T[j];
+                const int f = t >> (LOG2N - 3), b = t & ((N >> 3) - 1);
+                const int g = b >> (LL - 3), j = b & (E - 1);
+                jj[q] = j;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    x[q][r] = a[f * LD + col_at<V>(g * L + j + r * E)];
+                if constexpr (E > 1) {
+#pragma unroll
+                    for (int r = 1; r < 8; ++r)
+                        w[q][r - 1] = T[STEP * r * j];
+                }
             }
         }
 #pragma unroll
         for (int q = 0; q < IT; ++q) {
             const int t = tid + q * kThreads;
             if (FULL || t < TOT) {
-                const int f = t >> (LOG2N - 1), j = t & (H - 1);
-                const V ww = INV ? cconj(w[q]) : w[q];
-                a[f * LD + col_at<V>(j)] = cadd(x0[q], x1[q]);
-                a[f * LD + col_at<V>(j + H)] = cmul(csub(x0[q], x1[q]), ww);
+                const int f = t >> (LOG2N - 3), b = t & ((N >> 3) - 1);
+                const int g = b >> (LL - 3), j = jj[q];
+                V u[4], d[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    u[r] = cadd(x[q][r], x[q][r + 4]);
+                    d[r] = csub(x[q][r], x[q][r + 4]);
+                }
+                // d_r w_8^r: w_8 = (1 - i)/sqrt(2), w_8^2 = -i, w_8^3 = -(1 + i)/sqrt(2)
+                // (conjugates when INV)
+                if (INV) {
+                    d[1] = mk<V>((d[1].x - d[1].y) * h, (d[1].x + d[1].y) * h);
+                    d[2] = cmul_pi(d[2]);
+                    d[3] = mk<V>(-(d[3].x + d[3].y) * h, (d[3].x - d[3].y) * h);
+                } else {
+                    d[1] = mk<V>((d[1].x + d[1].y) * h, (d[1].y - d[1].x) * h);
+                    d[2] = cmul_mi(d[2]);
+                    d[3] = mk<V>((d[3].y - d[3].x) * h, -(d[3].x + d[3].y) * h);
+                }
+                V X8[8];
+                dft4<V, INV>(u[0], u[1], u[2], u[3], X8[0], X8[2], X8[4], X8[6]);
+                dft4<V, INV>(d[0], d[1], d[2], d[3], X8[1], X8[3], X8[5], X8[7]);
+                if constexpr (E > 1) {
+                    if (j > 0) {
+#pragma unroll
+                        for (int r = 1; r < 8; ++r)
+                            X8[r] = cmul(X8[r], INV ? cconj(w[q][r - 1]) : w[q][r - 1]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    a[f * LD + col_at<V>(g * L + j + r * E)] = X8[r];
             }
         }
         __syncthreads();
+        dif_radix8_ct<V, LOG2N, NBMAX, EXACT, INV, LL - 3>(a, nb, T);
     }
-    dif_radix4_ct<V, LOG2N, INV, LOG2N - (LOG2N & 1)>(a, T);
+}
+
+template <typename V, int LOG2N, int NBMAX, bool EXACT, bool INV>
+__device__ __forceinline__ void lds_fft_dif_ct(V* a, int nb, const V* T)
+{
+    if constexpr (dif_radix8(LOG2N)) {
+        dif_radix8_ct<V, LOG2N, NBMAX, EXACT, INV, LOG2N>(a, nb, T);
+    } else {
+        if constexpr (LOG2N & 1) {
+            constexpr int N = 1 << LOG2N, LD = col_ld<V>(N);
+            constexpr int H = N >> 1, IT = (NBMAX * H + kThreads - 1) / kThreads;
+            constexpr bool FULL = EXACT && (NBMAX * H) % kThreads == 0;
+            const int TOT = (EXACT ? NBMAX : nb) * H;
+            const int tid = threadIdx.x;
+            V x0[IT], x1[IT], w[IT];
+#pragma unroll
+            for (int q = 0; q < IT; ++q) {
+                const int t = tid + q * kThreads;
+                if (FULL || t < TOT) {
+                    const int f = t >> (LOG2N - 1), j = t & (H - 1);
+                    x0[q] = a[f * LD + col_at<V>(j)];
+                    x1[q] = a[f * LD + col_at<V>(j + H)];
+                    w[q] = T[j];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < IT; ++q) {
+                const int t = tid + q * kThreads;
+                if (FULL || t < TOT) {
+                    const int f = t >> (LOG2N - 1), j = t & (H - 1);
+                    const V ww = INV ? cconj(w[q]) : w[q];
+                    a[f * LD + col_at<V>(j)] = cadd(x0[q], x1[q]);
+                    a[f * LD + col_at<V>(j + H)] = cmul(csub(x0[q], x1[q]), ww);
+                }
+            }
+            __syncthreads();
+        }
+        dif_radix4_ct<V, LOG2N, NBMAX, EXACT, INV, LOG2N - (LOG2N & 1)>(a, nb, T);
+    }
 }
 
 // lds_fft_dif of a column tile of col_count(log2n) columns, through the
@@ -452,8 +561,8 @@ __device__ __forceinline__ void lds_fft_cols(V* a, int n, int log2n, int nb, int
 #define SMIP_DIF_CASE(LN)                                                                                             \
     case LN:                                                                                                          \
         if constexpr (col_count<V>(LN) > 0) {                                                                        \
-            lds_fft_dif_ct<V, LN, INV>(                                                                               \
-                a, STAGED && tw_staged<V>(1 << LN) ? reinterpret_cast<const V*>(at) : tw);                           \
+            lds_fft_dif_ct<V, LN, col_count<V>(LN), true, INV>(                                                            \
+                a, nb, STAGED && tw_staged<V>(1 << LN) ? reinterpret_cast<const V*>(at) : tw);                       \
             return;                                                                                                   \
         }                                                                                                             \
         break;
@@ -477,10 +586,46 @@ __device__ __forceinline__ void lds_fft_cols(V* a, int n, int log2n, int nb, int
 // Transforms per workgroup of a row pass when the frame has rows to spare
 // (rows_per_wg on the host): enough that a radix-4 stage has a butterfly for
 // each of the 256 lanes.
+// Row passes on the in-place DIF (one LDS buffer) for FP64, on the Stockham
+// ping-pong pair for FP32.
+template <typename V>
+__host__ __device__ constexpr bool rows_dif() { return sizeof(V) == 16; }
+
 __host__ __device__ constexpr int row_fill(int log2n)
 {
     return (4 * kThreads) >> log2n > 1 ? (4 * kThreads) >> log2n : 1;
 }
+
+// The in-place DIF on a row tile: nb <= row_fill(log2n) transforms of length n,
+// transform p at a[p * col_ld(n) + col_at(i)], through the compile-time form for
+// lengths 32..2048 (and for 64 and 512 points always, whose radix-8 output order
+// dif_pos assumes).  With STAGED, the twiddles sit at `at` in LDS when
+// tw_staged(n), else they are read from tw.
+template <typename V, bool INV, bool STAGED>
+__device__ __forceinline__ void lds_fft_rows_dif(V* a, int n, int log2n, int nb, const V* __restrict__ tw,
+    const char* at)
+{
+#define SMIP_ROWDIF_CASE(LN)                                                                                          \
+    case LN:                                                                                                          \
+        lds_fft_dif_ct<V, LN, row_fill(LN), false, INV>(                                                                     \
+            a, nb, STAGED && tw_staged<V>(1 << LN) ? reinterpret_cast<const V*>(at) : tw);                           \
+        return;
+    switch (log2n) {
+        SMIP_ROWDIF_CASE(5)
+        SMIP_ROWDIF_CASE(6)
+        SMIP_ROWDIF_CASE(7)
+        SMIP_ROWDIF_CASE(8)
+        SMIP_ROWDIF_CASE(9)
+        SMIP_ROWDIF_CASE(10)
+        SMIP_ROWDIF_CASE(11)
+    default:
+        break;
+    }
+#undef SMIP_ROWDIF_CASE
+    lds_fft_dif<V>(a, n, log2n, nb, col_ld<V>(n), STAGED && tw_staged<V>(n) ? reinterpret_cast<const V*>(at) : tw,
+        INV);
+}
+
 
 // lds_fft (Stockham, ping-pong) with the length, the transform count and the
 // direction fixed at compile time, as lds_fft_dif_ct is for the column form:
@@ -672,8 +817,14 @@ __device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int
     const V* __restrict__ tw, V* __restrict__ G, int hx, int absval, int bx, int64_t f, char* smem)
 {
     using S = typename scalar_of<V>::type;
+    // FP64 rows: one buffer, the in-place DIF (transform p at A[p * ld +
+    // col_at(i)], X[k] at dif_pos(k)): half the LDS of a ping-pong pair, so more
+    // workgroups fit a CU (the 512-point FP64 row passes of a 64-frame chunk took
+    // 49-51 us instead of 73).  FP32 rows keep the Stockham pair, which measured
+    // faster there.
+    constexpr bool kDif = rows_dif<V>();
+    const int ld = kDif ? col_ld<V>(X) : X;
     V* A = reinterpret_cast<V*>(smem);
-    V* B = A + ppw * X;
     const int pair0 = bx * ppw;
     const In* fr = frames + f * (int64_t)X * Y;
     gather_tile2<kRowPer, kRowPer, Pair<In>>(
@@ -683,18 +834,27 @@ __device__ __forceinline__ void rows_fwd_body(const In* __restrict__ frames, int
             const int64_t y0 = 2 * (pair0 + p);
             return Pair<In> { fr[y0 * X + x], fr[(y0 + 1) * X + x] };
         },
-        [&](int t, const Pair<In>& r) { A[t] = mk<V>(sample_value<S>(r.a, absval), sample_value<S>(r.b, absval)); });
-    char* at = smem + 2 * sizeof(V) * ppw * X;
+        [&](int t, const Pair<In>& r) {
+            const int at = kDif ? (t >> lx) * ld + col_at<V>(t & (X - 1)) : t;
+            A[at] = mk<V>(sample_value<S>(r.a, absval), sample_value<S>(r.b, absval));
+        });
+    char* at = smem + sizeof(V) * ppw * ld * (kDif ? 1 : 2);
     (void)stage_twiddles<V>(tw, X, at);
     __syncthreads();
-    const V* R = lds_fft_rows<V, false, true>(A, B, X, lx, ppw, tw, at);
+    const V* R = A;
+    if constexpr (kDif)
+        lds_fft_rows_dif<V, false, true>(A, X, lx, ppw, tw, at);
+    else
+        R = lds_fft_rows<V, false, true>(A, A + ppw * X, X, lx, ppw, tw, at);
+    // position of element k of transform p
+    auto pos = [&](int p, int k) { return kDif ? p * ld + col_at<V>(dif_pos(k, lx)) : p * X + k; };
     const S half = (S)0.5;
     // t / hx by a multiply with ceil(2^32 / hx): exact for t < 2^32 / hx
     const uint32_t hx_inv = 0xffffffffu / (uint32_t)hx + 1u;
     for (int t = threadIdx.x; t < ppw * hx; t += blockDim.x) {
         const int p = (int)__umulhi((uint32_t)t, hx_inv), k = t - p * hx;
-        const V zk = R[p * X + k];
-        const V zm = cconj(R[p * X + ((X - k) & (X - 1))]);
+        const V zk = R[pos(p, k)];
+        const V zm = cconj(R[pos(p, (X - k) & (X - 1))]);
         const V s = cadd(zk, zm), dlt = cmul_mi(csub(zk, zm));
         const int64_t y0 = 2 * (pair0 + p);
         G[(f * Y + y0) * hx + k] = mk<V>(s.x * half, s.y * half);
@@ -1018,8 +1178,9 @@ __device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__
 {
     __shared__ double wv[kThreads / 64];
     __shared__ int64_t wi[kThreads / 64];
+    // one buffer, as in rows_fwd_body: z(x) of pair p at A[p * ld + col_at(dif_pos(x))]
+    const int ld = col_ld<double2>(X);
     double2* A = reinterpret_cast<double2*>(smem);
-    double2* B = A + ppw * X;
     const int pair0 = bx * ppw;
     gather_tile2<kRowPer, 4, Pair<double2>>(
         ppw * X,
@@ -1030,23 +1191,25 @@ __device__ __forceinline__ void rows_c2r_argmax_body(const double2* __restrict__
             const int i = k > X / 2 ? X - k : k;
             return Pair<double2> { g0[i], g0[hx + i] };
         },
-        [&](int t, const Pair<double2>& r) { A[t] = c2r_pair_input(r.a, r.b, t & (X - 1), X); });
-    char* at = smem + 2 * sizeof(double2) * ppw * X;
+        [&](int t, const Pair<double2>& r) {
+            A[(t >> lx) * ld + col_at<double2>(t & (X - 1))] = c2r_pair_input(r.a, r.b, t & (X - 1), X);
+        });
+    char* at = smem + sizeof(double2) * ppw * ld;
     (void)stage_twiddles<double2>(tw, X, at);
     __syncthreads();
-    const double2* Z = lds_fft_rows<double2, true, true>(A, B, X, lx, ppw, tw, at);
+    lds_fft_rows_dif<double2, true, true>(A, X, lx, ppw, tw, at);
     // the scan starts from element 0 (max_element's first candidate) in the
     // workgroup that holds it, from nothing elsewhere
     double bv = -INFINITY;
     int64_t bi = INT64_MAX;
     if (bx == 0) {
-        bv = Z[0].x;
+        bv = A[0].x; // z(0) of pair 0: dif_pos(0) = 0
         bi = 0;
     }
     for (int t = threadIdx.x; t < ppw * X; t += blockDim.x) {
         const int p = t >> lx, x = t & (X - 1);
         const int64_t i0 = (2 * (int64_t)(pair0 + p)) * X + x; // Z.x is row y0, Z.y row y0 + 1
-        const double2 z = Z[t];
+        const double2 z = A[p * ld + col_at<double2>(dif_pos(x, lx))];
         better_of(bv, bi, z.x, i0);
         better_of(bv, bi, z.y, i0 + X);
     }
@@ -1338,6 +1501,10 @@ static int cols_per_wg(int n)
     int p = 1;
     while (p * 2 <= k)
         p *= 2;
+    // the column kernels take the compile-time transforms (and, for radix-8
+    // lengths, their output order) only for this count
+    if (p != col_count<V>(ilog2(n)))
+        fail(SMIP_ERR_LOGIC, "column count differs from the compiled transforms");
     return p;
 }
 
@@ -1354,6 +1521,16 @@ static int rows_per_wg(int n, int cap)
     return r;
 }
 
+// LDS of a row pass (rows_fwd_body, rows_c2r_argmax_body) and its staged
+// twiddles: ppw padded rows on the in-place DIF (FP64), a ping-pong pair of
+// ppw rows on the Stockham form (FP32)
+template <typename V>
+static size_t rows_lds_bytes(int X, int ppw)
+{
+    const size_t data = rows_dif<V>() ? sizeof(V) * (size_t)ppw * col_ld<V>(X) : 2 * sizeof(V) * (size_t)ppw * X;
+    return data + tw_lds_bytes<V>(X);
+}
+
 template <typename V>
 static void launch_rows_fwd(const FftPlan* p, const void* frames, int dtype, int64_t nf, bool absval, V* G,
     hipStream_t s)
@@ -1361,7 +1538,7 @@ static void launch_rows_fwd(const FftPlan* p, const void* frames, int dtype, int
     const int X = p->X, Y = p->Y, hx = X / 2 + 1;
     const int ppw = rows_per_wg<V>(X, Y / 2);
     dim3 grid((Y / 2) / ppw, (unsigned)nf);
-    const size_t lds = 2 * sizeof(V) * ppw * X + tw_lds_bytes<V>(X);
+    const size_t lds = rows_lds_bytes<V>(X, ppw);
     switch (dtype) {
     case SMIP_DTYPE_U8:
         hipLaunchKernelGGL((k_rows_fwd<V, uint8_t>), grid, dim3(kThreads), lds, s, (const uint8_t*)frames, X, Y, p->lx,
@@ -1460,7 +1637,7 @@ void launch_shift_f64(const FftPlan* p, const void* frames, int dtype, int64_t n
     const int ppw = rows_per_wg<double2>(X, Y / 2);
     const int nblk = (Y / 2) / ppw;
     hipLaunchKernelGGL(k_rows_c2r_argmax, dim3(nblk, (unsigned)nf), dim3(kThreads),
-        2 * sizeof(double2) * ppw * X + tw_lds_bytes<double2>(X), s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
+        rows_lds_bytes<double2>(X, ppw), s, work, X, Y, p->lx, ppw, hx, p->twx64, best);
     SMIP_HIP(hipGetLastError());
     launch_best_to_shift(best, nblk, nf, X, Y, shifts, s);
 }
@@ -1495,9 +1672,9 @@ FftTiling fft_tiling(const FftPlan* p)
     t.ppw64 = rows_per_wg<double2>(X, Y / 2);
     t.cb64 = cols_per_wg<double2>(Y);
     t.nblk64 = (Y / 2) / t.ppw64;
-    t.lds_rows32 = 2 * sizeof(float2) * t.ppw32 * X + tw_lds_bytes<float2>(X);
+    t.lds_rows32 = rows_lds_bytes<float2>(X, t.ppw32);
     t.lds_cols32 = sizeof(float2) * t.cb32 * col_ld<float2>(Y) + tw_lds_bytes<float2>(Y);
-    t.lds_rows64 = 2 * sizeof(double2) * t.ppw64 * X + tw_lds_bytes<double2>(X);
+    t.lds_rows64 = rows_lds_bytes<double2>(X, t.ppw64);
     t.lds_cols64 = sizeof(double2) * t.cb64 * col_ld<double2>(Y) + tw_lds_bytes<double2>(Y);
     return t;
 }
