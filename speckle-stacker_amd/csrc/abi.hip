@@ -1579,6 +1579,13 @@ void enqueue_saa(smip_job* j, const uint8_t* fm, int64_t m, int32_t* sh, hipStre
     }
 }
 
+// hipGraphLaunch under the library's capture lock, as capture_graph instantiates
+void graph_launch(hipGraphExec_t g, hipStream_t s)
+{
+    std::lock_guard<std::recursive_mutex> lk(capture_guard());
+    SMIP_HIP(hipGraphLaunch(g, s));
+}
+
 bool graphs_enabled()
 {
     const char* e = std::getenv("SMIP_NO_GRAPH");
@@ -1603,8 +1610,15 @@ hipGraphExec_t capture_graph(hipStream_t s, F&& enqueue)
         SMIP_HIP(hipStreamEndCapture(s, &g));
     }
     hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    hipError_t e;
+    {
+        // graph instantiation and launches are serialised (graph_launch): a host
+        // SIGSEGV inside hipGraphLaunch was once recorded right after another
+        // thread's instantiation
+        std::lock_guard<std::recursive_mutex> lk(capture_guard());
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+    }
     SMIP_HIP(e);
     return ex;
 }
@@ -1689,7 +1703,7 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
         if (graph) {
             if (!j->g_acc[ub])
                 j->g_acc[ub] = capture_graph(s, [&] { enqueue_accumulate(j, fr, nf); });
-            SMIP_HIP(hipGraphLaunch(j->g_acc[ub], s));
+            graph_launch(j->g_acc[ub], s);
             j->graph_launches++;
             note_h32_reader(j);
         } else if (overlap) {
@@ -1728,7 +1742,7 @@ void job_process(smip_job* j, const void* frames, uint64_t count)
             if (graph) {
                 if (!j->g_saa[ub])
                     j->g_saa[ub] = capture_graph(s, [&] { enqueue_saa(j, fr, nf, j->bshift.get(), s); });
-                SMIP_HIP(hipGraphLaunch(j->g_saa[ub], s));
+                graph_launch(j->g_saa[ub], s);
                 j->graph_launches++;
                 SMIP_HIP(hipMemcpyAsync(j->shifts.get() + 2 * j->nframes, j->bshift.get(), sizeof(int32_t) * 2 * nf,
                     hipMemcpyDeviceToDevice, s));
