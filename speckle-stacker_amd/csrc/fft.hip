@@ -7,11 +7,16 @@
 // Conventions are FFTW's: forward exp(-2 pi i k x / n), backward exp(+...),
 // both unnormalised.
 //
-// Each 1-D transform runs in LDS as a Stockham autosort sequence of radix-4
-// stages with a final radix-2 stage when log2(n) is odd; one workgroup holds
-// several transforms so every stage keeps all 256 lanes busy.  Real frames are
-// transformed two rows at a time (z = row0 + i row1) and split afterwards.
-// Twiddles come from tables computed on the host in double precision.
+// Each 1-D transform runs in LDS; one workgroup holds several transforms so
+// every stage keeps all 256 lanes busy.  Column passes, and the FP64 row passes,
+// use an in-place decimation-in-frequency form (one buffer; radix-8 stages at 64
+// and 512 points, else radix-4 stages after a radix-2 one when log2(n) is odd)
+// whose digit-reversed output order is absorbed where the result is read; the
+// FP32 row passes use a Stockham autosort pair of buffers (radix-4 stages, a
+// final radix-2 one).  Lengths 32..2048 run forms compiled for their length.
+// Real frames are transformed two rows at a time (z = row0 + i row1) and split
+// afterwards.  Twiddles come from tables computed on the host in double
+// precision.
 #include <algorithm>
 #include <cmath>
 #include <vector>
