@@ -18,6 +18,9 @@
 // afterwards.  Twiddles come from tables computed on the host in double
 // precision.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <vector>
 
@@ -58,10 +61,8 @@ static V* upload_twiddles(int n, hipStream_t stream)
     return d;
 }
 
-FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream)
+static FftPlan* fft_plan_build(int32_t ncols, int32_t nrows, hipStream_t stream)
 {
-    if (ncols < 2 || nrows < 2 || ncols > kMaxFftLength || nrows > kMaxFftLength)
-        fail(SMIP_ERR_INVALID_ARGUMENT, "frame extents must be in [2, 2048]");
     auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
     auto* p = new FftPlan;
     p->X = ncols;
@@ -83,9 +84,36 @@ FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream)
     return p;
 }
 
+// Plans are read-only once built, so one per (device, ncols, nrows) is kept and
+// shared by every job and correlator of that shape: a job then starts without
+// the four synchronous twiddle uploads.  Up to kCachedPlans are kept (a few
+// hundred KiB of tables at most); beyond that a plan is built for its owner and
+// destroyed with it.  Like the other caches, kept plans are never destroyed.
+FftPlan* fft_plan_create(int32_t ncols, int32_t nrows, hipStream_t stream)
+{
+    if (ncols < 2 || nrows < 2 || ncols > kMaxFftLength || nrows > kMaxFftLength)
+        fail(SMIP_ERR_INVALID_ARGUMENT, "frame extents must be in [2, 2048]");
+    constexpr size_t kCachedPlans = 64;
+    int dev = 0;
+    SMIP_HIP(hipGetDevice(&dev));
+    static std::mutex mu;
+    static auto* cache = new std::map<std::tuple<int, int32_t, int32_t>, FftPlan*>;
+    const auto key = std::make_tuple(dev, ncols, nrows);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache->find(key);
+    if (it != cache->end())
+        return it->second;
+    FftPlan* p = fft_plan_build(ncols, nrows, stream);
+    if (cache->size() < kCachedPlans) {
+        p->cached = true;
+        (*cache)[key] = p;
+    }
+    return p;
+}
+
 void fft_plan_destroy(FftPlan* p)
 {
-    if (!p)
+    if (!p || p->cached)
         return;
     dev_free(p->twx32);
     dev_free(p->twy32);

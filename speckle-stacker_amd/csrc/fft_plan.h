@@ -28,6 +28,7 @@ struct FftPlan {
     float2* twy32 = nullptr;
     double2* twx64 = nullptr;
     double2* twy64 = nullptr;
+    bool cached = false; // kept by fft_plan_create for later jobs: fft_plan_destroy leaves it
 };
 
 // The launchers of fft.hip, for a plan with pow2 == false.
