@@ -1772,15 +1772,24 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
 {
     if (n <= 0)
         return;
+    // The table holds the jobs by window count, most first, so the jobs that still
+    // have window `win` are a prefix of it: each window's launches cover those
+    // jobs only, instead of every job with workgroups that return at once (each
+    // empty fixup workgroup still waits for a CU with its full LDS free).
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k)
+        order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x]->nwin > w[y]->nwin; });
     std::vector<RecoJob> tab(n);
     int64_t maxwin = 0;
-    for (int k = 0; k < n; ++k) {
+    for (int i = 0; i < n; ++i) {
+        const int k = order[i];
         const RecoArgs& a = w[k]->args;
         const int npts = a.X * a.Y;
         hipLaunchKernelGGL(k_reco_init, dim3((npts + 255) / 256), dim3(256), 0, s, a, phases[k], flags[k],
             consistency[k], w[k]->map);
         hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, w[k]->status);
-        RecoJob& J = tab[k];
+        RecoJob& J = tab[i];
         J.a = a;
         J.visits = w[k]->visits;
         J.first_pos = w[k]->first_pos;
@@ -1803,15 +1812,18 @@ void run_reconstruct_many(RecoWork* const* w, int n, const void* const* B, const
     RecoJob* dtab = static_cast<RecoJob*>(dev_malloc(sizeof(RecoJob) * n));
     try {
         copy_sync(dtab, tab.data(), sizeof(RecoJob) * n, hipMemcpyHostToDevice, s);
+        int na = n; // jobs with window `win`: a prefix of the table
         for (int64_t win = 0; win < maxwin; ++win) {
+            while (na > 0 && w[order[na - 1]]->nwin <= win)
+                --na;
             if (win == 0) // later windows' maps are written by the fixups before them
                 hipLaunchKernelGGL(k_reco_map_many, dim3(1, n), dim3(kWindow), 0, s, dtab, win);
             if (push) {
-                hipLaunchKernelGGL(k_reco_bulk_many<true>, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
-                hipLaunchKernelGGL(k_reco_fixup_many<true>, dim3(n), dim3(kPushThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_bulk_many<true>, dim3(kWindow, na), dim3(kBulkThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_fixup_many<true>, dim3(na), dim3(kPushThreads), 0, s, dtab, win);
             } else {
-                hipLaunchKernelGGL(k_reco_bulk_many<false>, dim3(kWindow, n), dim3(kBulkThreads), 0, s, dtab, win);
-                hipLaunchKernelGGL(k_reco_fixup_many<false>, dim3(n), dim3(kGenThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_bulk_many<false>, dim3(kWindow, na), dim3(kBulkThreads), 0, s, dtab, win);
+                hipLaunchKernelGGL(k_reco_fixup_many<false>, dim3(na), dim3(kGenThreads), 0, s, dtab, win);
             }
         }
         SMIP_HIP(hipGetLastError());
