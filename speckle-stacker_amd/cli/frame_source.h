@@ -508,6 +508,15 @@ private:
             throw std::runtime_error("PNG: missing IHDR or IDAT: " + path);
         if (ctype == 3 && (plte.empty() || plte.size() % 3))
             throw std::runtime_error("PNG: palette image without a valid PLTE: " + path);
+        // Extents are checked before any frame buffer is sized from them: at most
+        // 2^30 pixels (OpenCV's default CV_IO_MAX_IMAGE_PIXELS, the limit of the
+        // reference's cv::VideoCapture path), and no more pixel bytes than the
+        // compressed data can hold (deflate expands by at most 1032:1).
+        if ((uint64_t)w * h > (uint64_t(1) << 30))
+            throw std::runtime_error("PNG: image larger than 2^30 pixels: " + path);
+        const uint64_t channels = ctype == 2 ? 3 : ctype == 4 ? 2 : ctype == 6 ? 4 : 1;
+        if ((uint64_t)w * h * channels * depth / 8 > (uint64_t)z.size() * 1032 + 64)
+            throw std::runtime_error("PNG: image data too short for its extents: " + path);
         return z;
     }
 
